@@ -1,0 +1,185 @@
+"""Throughput benchmark: dense spherical parallel tempering on one MI355X.
+
+Workload (the reference's `parallel_tempering -N 64 -NT 32 -nrep 8 -Tmin 0.2
+-Tmax 1.6 -log_temp -pt_freq 1 -fmc 0 -seed 42`): N=64 fully connected
+(C(64,4) = 635,376 quartets, real couplings, one channel each), 32 geometric
+temperatures x 8 ladders = 256 replicas, an exchange after every sweep.
+A step is one sweep of all 256 replicas (N/2 = 32 pair proposals each) plus
+its exchange round, all on the GPU.
+
+Reports one JSON line: pair-updates/s (whole job, timed by wall clock around
+K steps after W warm-up steps, inputs resident in HBM), energy ESS/s at the
+coldest slot, the sweep kernel's roofline figure (HIP events around every
+sweep launch on the library's stream) and a CPU baseline (the oracle's
+OpenMP transcription of the reference sweep, FP64) on a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def n_inv(N):
+    return 2 * (N - 2) * (N - 3) * (N - 4) // 6 + (N - 2) * (N - 3) // 2
+
+
+def bytes_per_pair_update(N, h2_active, b_q=4, b_2=4):
+    """Algorithmic bytes of one pair-update: each touched quartet's coupling
+    (b_q bytes, real FP32) plus a 1-byte channel code, plus two g2 rows."""
+    return n_inv(N) * (b_q + 1) + (2 * N * b_2 if h2_active else 0)
+
+
+def tau_int_sokal(x, c=5.0):
+    """Integrated autocorrelation time with Sokal's automatic window."""
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x)
+    if n < 8:
+        return float("nan")
+    x = x - x.mean()
+    f = np.fft.rfft(x, 2 * n)
+    acf = np.fft.irfft(f * np.conj(f))[:n]
+    if acf[0] <= 0:
+        return float("nan")
+    rho = acf / acf[0]
+    tau = 1.0
+    for M in range(1, n):
+        tau += 2.0 * rho[M]
+        if M >= c * tau:
+            break
+    return max(tau, 1.0)
+
+
+def ess_of_series(series):
+    """series [rounds, ladders]: drop the first half, sum n/tau over ladders."""
+    half = series.shape[0] // 2
+    prod = series[half:]
+    ess = 0.0
+    taus = []
+    for r in range(prod.shape[1]):
+        t = tau_int_sokal(prod[:, r])
+        taus.append(t)
+        if np.isfinite(t):
+            ess += prod.shape[0] / t
+    return ess, taus
+
+
+def cpu_baseline(st, N, nrep, betas, seed, n_sweeps):
+    import oracle as O
+
+    O.build()
+    L = O.lib()
+    g2 = np.ascontiguousarray(st.g2(), dtype=np.float64).reshape(-1)
+    g4, mask = st.g4()
+    s = st.spins()
+    spins = np.ascontiguousarray(np.stack([s.real, s.imag], -1), dtype=np.float64).reshape(-1)
+    seeds, _ = st.rng_info()
+    E = np.zeros(nrep)
+    acc = np.zeros(nrep, dtype=np.int64)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    if os.environ.get("OMP_NUM_THREADS"):
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+    secs = L.or_cpu_baseline_sweeps(N, nrep, spins, g2, int(st.info.h2_active), g4, mask,
+                                    np.ascontiguousarray(betas), seeds, 1 << 40, n_sweeps, E, acc, cores)
+    pu = n_sweeps * nrep * (N // 2)
+    return {"value": pu / secs, "unit": "pair-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{n_sweeps} sweep(s) x {nrep} replicas of the same N={N} system "
+                      f"({pu} pair-updates, {secs:.1f} s, FP64, reference 3-type enumeration)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16384)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--N", type=int, default=64)
+    ap.add_argument("--NT", type=int, default=32)
+    ap.add_argument("--nrep", type=int, default=8)
+    ap.add_argument("--fmc", type=int, default=0)
+    ap.add_argument("--gamma", type=float, default=0.0)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--kernel", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--cpu-sweeps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(independent ladders would shard with no communication). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import cuda_pspin_amd as P
+
+    N, NT, nr = args.N, args.NT, args.nrep
+    nrep = NT * nr
+    Tmin, Tmax = 0.2, 1.6
+    A = (Tmin / Tmax) ** (1.0 / (NT - 1))
+    T = np.array([Tmax * A ** t for t in range(NT)])
+    beta = 1.0 / T
+    cfg = P.SimConfig.default(N=N, nrep=nrep, seed=args.seed, fmc_mode=args.fmc, gamma=args.gamma, dev=0)
+    t0 = time.time()
+    st = P.PSpin(cfg, P.DENSE)
+    st.set_kernel(args.kernel, args.block)
+    t_init = time.time() - t0
+    st.pt_init(NT, nr, beta, args.seed + 5000)
+    st.pt_run(args.warmup, 1)
+    st.pt_enable_series(args.steps)
+    st.set_profiling(True)
+    t0 = time.perf_counter()
+    st.pt_run(args.steps, 1)  # synchronises before returning
+    wall = time.perf_counter() - t0
+    prof = st.profile()
+    st.set_profiling(False)
+    series = st.pt_series()
+    ess, taus = ess_of_series(series)
+
+    pu_per_step = nrep * (N // 2)
+    value = args.steps * pu_per_step / wall
+    kern_ms = prof["sweep_ms"] / max(prof["n_sweep"], 1)
+    bpu = bytes_per_pair_update(N, st.info.h2_active)
+    achieved = pu_per_step * bpu / (kern_ms * 1e-3)
+    out = {
+        "metric": "pair-updates/s (dense PT, N=64, 1 MI355X)",
+        "value": value,
+        "unit": "pair-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded disorder and spins)",
+        "config": {"workload": f"dense spherical PT N={N} fmc={args.fmc} NT={NT} x nrep={nr} "
+                               f"T in [{Tmin},{Tmax}] geometric, exchange every sweep",
+                   "N": N, "NT": NT, "nrep": nr, "replicas": nrep, "pt_freq": 1, "fmc": args.fmc,
+                   "sweeps_timed": args.steps, "kernel": st.info.kernel},
+        "energy_ess_per_s_tmin": ess / wall,
+        "energy_tau_int_tmin_sweeps": float(np.nanmean(taus)) if taus else None,
+        "sweep_kernel_ms": kern_ms,
+        "exchange_kernel_ms": prof["exch_ms"] / max(prof["n_exch"], 1),
+        "init_s": t_init,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "bytes_per_pair_update": bpu, "pair_updates_per_launch": pu_per_step},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(st, N, nrep, np.repeat(beta, nr), args.seed, args.cpu_sweeps)
+    st.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,102 @@
+// exchange.hip -- parallel-tempering replica exchange on gfx950 (the
+// reference does it on the host between sweeps, src/parallel_tempering.cu:316-352).
+//
+// Decisions are identical to the reference's for the same energies and the
+// same drand48 stream: temperature pairs t = 0..NT-2 in order (hot to cold),
+// within each t the ladders r = 0..nrep-1 in order, delta =
+// (beta_t - beta_{t+1}) (E_a - E_b), accept when delta >= 0, otherwise draw
+// u = drand48() and accept when u < exp(delta).  One wavefront serves 64
+// ladders per pass: a ballot over "needs a draw" gives each lane its rank in
+// the reference's draw order, and lcg48_skip jumps the 48-bit state to it.
+#include "pspin_internal.h"
+
+namespace ps {
+
+struct ExArgs {
+    int NT, nrep;
+    const double* beta_sched;
+    const double* E;
+    int* perm;
+    double* betas;
+    long long* ex;  // [4][NT-1]: acc_win, prop_win, acc_tot, prop_tot
+    uint64_t* rng48;
+    double* series;
+    long long* series_n;
+    int series_cap;
+};
+
+__global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
+    extern __shared__ int sperm[];
+    const int lane = threadIdx.x;
+    const int NT = a.NT, nr = a.nrep, n = NT * nr;
+    for (int i = lane; i < n; i += 64) sperm[i] = a.perm[i];
+    __syncthreads();
+    uint64_t X = *a.rng48;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int t = 0; t < NT - 1; t++) {
+        const double db = a.beta_sched[t] - a.beta_sched[t + 1];
+        long long nacc = 0;
+        for (int c0 = 0; c0 < nr; c0 += 64) {
+            const int r = c0 + lane;
+            const bool valid = r < nr;
+            double delta = 0.0;
+            if (valid) delta = db * (a.E[sperm[t * nr + r]] - a.E[sperm[(t + 1) * nr + r]]);
+            const bool need = valid && !(delta >= 0.0);
+            const unsigned long long m = __ballot(need);
+            bool accept = valid;
+            if (need) {
+                const uint64_t x = lcg48_skip(X, (uint64_t)__popcll(m & lt) + 1);
+                accept = lcg48_unit(x) < exp(delta);
+            }
+            X = lcg48_skip(X, (uint64_t)__popcll(m));
+            if (valid && accept) {
+                const int tmp = sperm[t * nr + r];
+                sperm[t * nr + r] = sperm[(t + 1) * nr + r];
+                sperm[(t + 1) * nr + r] = tmp;
+            }
+            nacc += __popcll(__ballot(valid && accept));
+            __syncthreads();
+        }
+        if (lane == 0) {
+            a.ex[0 * (NT - 1) + t] += nacc;
+            a.ex[1 * (NT - 1) + t] += nr;
+            a.ex[2 * (NT - 1) + t] += nacc;
+            a.ex[3 * (NT - 1) + t] += nr;
+        }
+    }
+    __syncthreads();
+    for (int i = lane; i < n; i += 64) {
+        a.perm[i] = sperm[i];
+        a.betas[sperm[i]] = a.beta_sched[i / nr];
+    }
+    if (a.series) {
+        const long long round = *a.series_n;
+        double* row = a.series + (size_t)(round % a.series_cap) * nr;
+        for (int r = lane; r < nr; r += 64) row[r] = a.E[sperm[(NT - 1) * nr + r]];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        *a.rng48 = X;
+        if (a.series) *a.series_n += 1;
+    }
+}
+
+hipError_t launch_exchange(pspin_state* st) {
+    ExArgs a;
+    a.NT = st->pt_NT;
+    a.nrep = st->pt_nrep;
+    a.beta_sched = st->d_beta_sched;
+    a.E = st->d_E;
+    a.perm = st->d_perm;
+    a.betas = st->d_betas;
+    a.ex = st->d_ex;
+    a.rng48 = st->d_rng48;
+    a.series = st->series_cap > 0 ? st->d_series : nullptr;
+    a.series_n = st->d_series_n;
+    a.series_cap = st->series_cap;
+    size_t lds = (size_t)a.NT * a.nrep * sizeof(int);
+    hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), lds, st->stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace ps

@@ -1,0 +1,223 @@
+// pspin_internal.h -- shared device helpers and the library state behind
+// the C ABI in include/pspin.h.  gfx950 (CDNA4, wave64) only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/pspin.h"
+
+#define PS_HD __host__ __device__ __forceinline__
+
+namespace ps {
+
+constexpr int kMaxN = 256;         // site indices are packed in 8 bits
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr uint32_t kTagSpins = 1;  // Philox stream tags (second key word is the seed)
+constexpr uint32_t kTagG2 = 2;
+constexpr uint32_t kTagG4 = 4;
+
+// ---------------------------------------------------------------------------
+// splitmix64 (Steele, Lea, Flood 2014) -- the reference's seed chain
+// (src/mc.cu:42-47).
+// ---------------------------------------------------------------------------
+PS_HD uint64_t splitmix64(uint64_t* s) {
+    uint64_t z = (*s += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): counter-based, so every
+// (replica, sweep, step) draw is addressable without stored RNG state.
+// ---------------------------------------------------------------------------
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+PS_HD U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        U4 n;
+        n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+        n.y = (uint32_t)p1;
+        n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+        n.w = (uint32_t)p0;
+        c = n;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+PS_HD double u32_to_unit(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
+PS_HD double u64_to_unit(uint32_t hi, uint32_t lo) {
+    uint64_t m = ((uint64_t)(hi >> 5) << 26) | (uint64_t)(lo >> 6);
+    return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Standard normal number k of stream (seed, tag): Box-Muller on two 53-bit
+// uniforms from one Philox block per pair of normals.
+__device__ __forceinline__ double normal_k(uint64_t seed, uint32_t tag, uint64_t k) {
+    uint64_t pair = k >> 1;
+    U4 x = philox4x32_10(U4{(uint32_t)pair, (uint32_t)(pair >> 32), 0u, tag}, (uint32_t)seed,
+                         (uint32_t)(seed >> 32));
+    double u1 = u64_to_unit(x.x, x.y), u2 = u64_to_unit(x.z, x.w);
+    double r = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincos(2.0 * M_PI * u2, &s, &c);
+    return (k & 1) ? r * s : r * c;
+}
+
+// ---------------------------------------------------------------------------
+// drand48 (POSIX): X' = (0x5DEECE66D X + 0xB) mod 2^48, value X'/2^48;
+// srand48(s) sets X = (low 32 bits of s) << 16 | 0x330E.  lcg48_skip applies
+// k steps at once so a wavefront can serve several draws in parallel.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kLcgA = 0x5DEECE66DULL;
+constexpr uint64_t kLcgC = 0xBULL;
+constexpr uint64_t kMask48 = (1ULL << 48) - 1;
+
+PS_HD uint64_t srand48_state(long s) { return (((uint64_t)(uint32_t)s) << 16) | 0x330EULL; }
+
+PS_HD uint64_t lcg48_skip(uint64_t x, uint64_t k) {
+    uint64_t am = 1, ap = 0, cm = kLcgA, cp = kLcgC;
+    while (k) {
+        if (k & 1) {
+            am = am * cm;
+            ap = ap * cm + cp;
+        }
+        cp = (cm + 1) * cp;
+        cm = cm * cm;
+        k >>= 1;
+    }
+    return (am * x + ap) & kMask48;
+}
+
+PS_HD double lcg48_unit(uint64_t x) { return (double)x * (1.0 / 281474976710656.0); }
+
+PS_HD long long n_quartets(int N) { return (long long)N * (N - 1) * (N - 2) * (N - 3) / 24; }
+PS_HD long long n_pairs(int N) { return (long long)N * (N + 1) / 2; }
+
+// Conjugation masks per channel (bit p set = sorted position p conjugated),
+// src/mc.cu:119-121.
+PS_HD int conj_mask(int ch) { return ch == 0 ? 0xC : (ch == 1 ? 0x6 : 0xA); }
+
+// One step of a sweep, precomputed for all steps at sweep start.
+struct StepPre {
+    int i0, j0;
+    double c[6];  // sphere: cos a, sin a, cos p1, sin p1, cos p2, sin p2; cube: sqrt(1-u), sqrt(u), phases
+    double thr;   // accept iff dE < thr (thr = -log(u)/beta; +inf when beta == 0)
+};
+
+// Per-step log record (48 bytes), see pspin_enable_step_log.
+struct StepRec {
+    int i0, j0, acc;
+    float dE;
+    double ai[2], aj[2];
+};
+
+}  // namespace ps
+
+// ---------------------------------------------------------------------------
+// Library state (the MCState / MCStateSparse of include/mc.h:9-30 and
+// include/mc_sparse.h:15-36, extended with the PT state that the reference
+// keeps in its driver).
+// ---------------------------------------------------------------------------
+struct pspin_state {
+    pspin_sim_config cfg{};
+    int variant = PSPIN_DENSE;
+    int N = 0, nrep = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    int h2_active = 0;
+    long long n_pairs_active = 0, n_quart_active = 0;
+    std::vector<double> h_omega;
+
+    // disorder
+    double* d_g2_64 = nullptr;  // [N*N] real, symmetric
+    float* d_g2_32 = nullptr;   // [N*N]
+    double* d_g4_64 = nullptr;  // [C(N,4)] dense only
+    uint8_t* d_mask = nullptr;  // [C(N,4)] dense only
+    int n_sq = 0;               // sparse quartet count
+    int* d_sq = nullptr;        // [n_sq*5] (i,j,k,l,ch)
+    double* d_sqg = nullptr;    // [n_sq]
+    std::vector<int> h_sq;
+    std::vector<double> h_sqg;
+
+    // per-site H4 lists used by the sweep: entry = {p | m<<8 | n<<16, g (f32 bits)}
+    int* d_site_off = nullptr;      // [N+1]
+    uint2* d_site_ent = nullptr;    // [n_ent]
+    long long n_ent = 0;
+    int max_site_len = 0;
+
+    // replicas
+    double* d_spins = nullptr;  // [nrep][N][2]
+    double* d_E = nullptr;
+    double* d_betas = nullptr;
+    long long* d_acc = nullptr;
+    long long* d_prop = nullptr;
+    uint64_t* d_seeds = nullptr;
+    std::vector<uint64_t> h_seeds;
+    uint64_t sweep_counter = 0;
+    double* d_e2 = nullptr;
+    double* d_e4 = nullptr;
+
+    // parallel tempering
+    int pt_NT = 0, pt_nrep = 0;
+    double* d_beta_sched = nullptr;
+    int* d_perm = nullptr;
+    long long* d_ex = nullptr;  // [4][NT-1]: acc_win, prop_win, acc_tot, prop_tot
+    uint64_t* d_rng48 = nullptr;
+    long long pt_sweeps = 0;
+    double* d_series = nullptr;
+    int series_cap = 0;
+    long long* d_series_n = nullptr;
+
+    // step log
+    ps::StepRec* d_log = nullptr;
+    int log_cap = 0;
+    int log_n = 0;
+
+    // kernel choice and profiling
+    int kernel = PSPIN_KERNEL_LISTS;
+    int block_size = 0;
+    bool prof = false;
+    std::vector<hipEvent_t> ev_free;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_sweep, ev_exch;
+    double prof_sweep_ms = 0.0, prof_exch_ms = 0.0;
+    long long prof_n_sweep = 0, prof_n_exch = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Internal host launchers (one translation unit per kernel family).
+// All launch on st->stream and return hipError_t.
+// ---------------------------------------------------------------------------
+namespace ps {
+void set_error(const std::string& msg);
+
+hipError_t launch_g2_build(double* g2, int N, uint64_t seed, int fmc_on, const double* d_omega, double gamma,
+                           hipStream_t s);
+hipError_t launch_g4_build(double* g4, uint8_t* mask, int N, uint64_t seed, int fmc_mode, const double* d_omega,
+                           double gamma, hipStream_t s);
+hipError_t launch_scale_shift(double* g, long long n, double sigma, double mean, hipStream_t s);
+hipError_t launch_to_f32(const double* src, float* dst, long long n, hipStream_t s);
+hipError_t launch_init_spins(double* spins, int N, int nrep, const uint64_t* seeds, int cube, hipStream_t s);
+
+hipError_t launch_energy_dense(const pspin_state* st, double* e2, double* e4);
+hipError_t launch_energy_sparse(const pspin_state* st, double* e2, double* e4);
+hipError_t launch_add(const double* a, const double* b, double* out, int n, hipStream_t s);
+
+hipError_t launch_sweep(pspin_state* st, int n_sweeps);
+
+hipError_t launch_exchange(pspin_state* st);
+}  // namespace ps

@@ -1,0 +1,268 @@
+// sweep.hip -- fused Monte Carlo sweep on gfx950 (the reference's
+// mc_sweep_kernel, src/mc.cu:66-318, and mc_sparse_sweep_kernel,
+// src/mc_sparse.cu:63-253, re-designed).
+//
+// One workgroup owns one replica and makes N/2 pair proposals per sweep.
+//
+// Delta E by the midpoint rule.  Every Hamiltonian term is a real
+// multilinear form in the amplitudes of distinct sites, so for a move
+// (a_i, a_j) -> (a_i', a_j') with d_x = a_x' - a_x and midpoints
+// m_x = (a_x + a_x')/2 it is exact that
+//     Delta E = -Re[d_i F_i(s~)] - Re[d_j F_j(s~)],
+// where s~ is the configuration with a_i, a_j replaced by their midpoints and
+// F_x is the local field of site x (the coefficient of a_x in -E):
+//     F_x = sum_{k != x} g2_xk conj(s~_k) + sum_{q containing x} g_q X_{x,q}(s~).
+// Writing each H4 term as -g Re[a_x X] with X = a_p conj(a_m) conj(a_n)
+// (p = the position sharing x's conjugation in the channel mask of
+// src/mc.cu:119-121, m, n the other two) covers the reference's three
+// enumeration types (src/mc.cu:111-275) with no special case for quartets
+// that contain both sites, and in differential form, so FP32 arithmetic does
+// not cancel new-minus-old products.
+//
+// Couplings of site x are a contiguous list of (p | m<<8 | n<<16, g) entries:
+// coalesced 8-byte loads; the three amplitudes come from LDS.
+//
+// Randomness: Philox4x32-10 keyed by the replica seed (the reference's
+// splitmix64 chain, src/mc.cu:418-423) and countered by (sweep, step), all N/2
+// steps drawn in parallel at sweep start.  Proposal arithmetic and the master
+// amplitudes are FP64 (the constraint is kept to FP64 rounding, as in
+// the reference); Delta E is FP32; the running energy is FP64.
+#include "pspin_internal.h"
+
+namespace ps {
+
+struct SweepArgs {
+    int N, nrep, h2_active, n_sweeps;
+    double* spins;
+    const float* g2;
+    const int* site_off;
+    const uint2* site_ent;
+    const double* betas;
+    double* E;
+    long long* acc;
+    long long* prop;
+    const uint64_t* seeds;
+    uint64_t sweep0;
+    StepRec* log;
+    int log_cap;   // sweeps per replica in the log
+    int log_base;  // sweeps already logged
+};
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Draws and trigonometry for one step (pair pick src/mc.cu:129-141; angles
+// include/spins.h:25-32 or include/spins_sparse.h:34-41).
+template <bool CUBE>
+__device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t sweep, int step, int N, double beta) {
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 0u}, k0, k1);
+    U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 1u}, k0, k1);
+    StepPre p;
+    double u1 = u32_to_unit(x.x), u2 = u32_to_unit(x.y);
+    int ri = (int)(u1 * N);
+    if (ri >= N) ri = N - 1;
+    int rj = (int)(u2 * (N - 1));
+    if (rj >= N - 1) rj = N - 2;
+    if (rj >= ri) rj++;
+    p.i0 = ri < rj ? ri : rj;
+    p.j0 = ri < rj ? rj : ri;
+    double ua = u32_to_unit(x.z), up1 = u32_to_unit(x.w), up2 = u32_to_unit(y.x), uacc = u32_to_unit(y.y);
+    if (CUBE) {
+        p.c[0] = sqrt(1.0 - ua);
+        p.c[1] = sqrt(ua);
+    } else {
+        sincos(2.0 * M_PI * ua, &p.c[1], &p.c[0]);
+    }
+    sincos(2.0 * M_PI * up1, &p.c[3], &p.c[2]);
+    sincos(2.0 * M_PI * up2, &p.c[5], &p.c[4]);
+    // u < exp(-beta dE)  <=>  dE < -log(u)/beta  (src/mc.cu:289-296)
+    p.thr = (beta == 0.0) ? INFINITY : -log(uacc) / beta;
+    return p;
+}
+
+template <bool CUBE>
+__global__ __launch_bounds__(1024) void k_sweep_lists(SweepArgs a) {
+    __shared__ double2 s64[kMaxN];
+    __shared__ float2 s32[kMaxN];
+    __shared__ StepPre pre[kMaxN / 2];
+    __shared__ float part[2][16];
+    __shared__ double2 newv[2];
+    __shared__ float2 dlt[2];
+    __shared__ int pair[2];
+
+    const int rep = blockIdx.x;
+    const int tid = threadIdx.x, bdim = blockDim.x;
+    const int nw = bdim >> 6, w = tid >> 6, lane = tid & 63;
+    const int N = a.N, nsteps = N / 2;
+    double* gs = a.spins + (size_t)2 * N * rep;
+
+    for (int i = tid; i < N; i += bdim) {
+        double2 v = make_double2(gs[2 * i], gs[2 * i + 1]);
+        s64[i] = v;
+        s32[i] = make_float2((float)v.x, (float)v.y);
+    }
+    const double beta = a.betas[rep];
+    const uint64_t seed = a.seeds[rep];
+    double E = 0.0;
+    long long nacc = 0, nprop = 0;
+    if (tid == 0) E = a.E[rep];
+
+    for (int sw = 0; sw < a.n_sweeps; sw++) {
+        const uint64_t sweep = a.sweep0 + (uint64_t)sw;
+        for (int k = tid; k < nsteps; k += bdim) pre[k] = make_step<CUBE>(seed, sweep, k, N, beta);
+        __syncthreads();
+        StepRec* logp = nullptr;
+        if (a.log && a.log_base + sw < a.log_cap)
+            logp = a.log + ((size_t)rep * a.log_cap + a.log_base + sw) * nsteps;
+
+        for (int step = 0; step <= nsteps; step++) {
+            if (tid == 0) {
+                if (step > 0) {
+                    // Resolve the previous proposal (src/mc.cu:284-304).
+                    const int sp = step - 1, pi = pair[0], pj = pair[1];
+                    float dE = 0.f;
+                    for (int k = 0; k < nw; k++) dE += part[sp & 1][k];
+                    const bool accept = (beta == 0.0) || ((double)dE < pre[sp].thr);
+                    if (accept) {
+                        s64[pi] = newv[0];
+                        s64[pj] = newv[1];
+                        E += (double)dE;
+                        nacc++;
+                    }
+                    s32[pi] = make_float2((float)s64[pi].x, (float)s64[pi].y);
+                    s32[pj] = make_float2((float)s64[pj].x, (float)s64[pj].y);
+                    nprop++;
+                    if (logp) {
+                        StepRec r;
+                        r.i0 = pi; r.j0 = pj; r.acc = accept; r.dE = dE;
+                        r.ai[0] = newv[0].x; r.ai[1] = newv[0].y;
+                        r.aj[0] = newv[1].x; r.aj[1] = newv[1].y;
+                        logp[sp] = r;
+                    }
+                }
+                if (step < nsteps) {
+                    // Constraint-preserving proposal (include/spins.h:17-52,
+                    // include/spins_sparse.h:26-56), FP64.
+                    const StepPre& P = pre[step];
+                    const int i0 = P.i0, j0 = P.j0;
+                    const double2 ai = s64[i0], aj = s64[j0];
+                    const double r1 = ai.x * ai.x + ai.y * ai.y, r2 = aj.x * aj.x + aj.y * aj.y;
+                    double2 ni, nj;
+                    if (CUBE) {
+                        const double sq = sqrt(r1 * r1 + r2 * r2);
+                        const double m1 = sqrt(sq * P.c[0]), m2 = sqrt(sq * P.c[1]);
+                        ni = make_double2(m1 * P.c[2], m1 * P.c[3]);
+                        nj = make_double2(m2 * P.c[4], m2 * P.c[5]);
+                    } else {
+                        const double f1 = sqrt((r1 + r2) / r1) * P.c[0], f2 = sqrt((r1 + r2) / r2) * P.c[1];
+                        ni = make_double2((ai.x * P.c[2] + ai.y * P.c[3]) * f1, (-ai.x * P.c[3] + ai.y * P.c[2]) * f1);
+                        nj = make_double2((aj.x * P.c[4] + aj.y * P.c[5]) * f2, (-aj.x * P.c[5] + aj.y * P.c[4]) * f2);
+                    }
+                    newv[0] = ni;
+                    newv[1] = nj;
+                    dlt[0] = make_float2((float)(ni.x - ai.x), (float)(ni.y - ai.y));
+                    dlt[1] = make_float2((float)(nj.x - aj.x), (float)(nj.y - aj.y));
+                    s32[i0] = make_float2((float)(0.5 * (ai.x + ni.x)), (float)(0.5 * (ai.y + ni.y)));
+                    s32[j0] = make_float2((float)(0.5 * (aj.x + nj.x)), (float)(0.5 * (aj.y + nj.y)));
+                    pair[0] = i0;
+                    pair[1] = j0;
+                }
+            }
+            __syncthreads();
+            if (step == nsteps) break;
+
+            const int i0 = pair[0], j0 = pair[1];
+            const float2 di = dlt[0], dj = dlt[1];
+            float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
+            if (a.h2_active) {
+                const float* ri = a.g2 + (size_t)i0 * N;
+                const float* rj = a.g2 + (size_t)j0 * N;
+                for (int k = tid; k < N; k += bdim) {
+                    const float2 t = s32[k];
+                    if (k != i0) { const float g = ri[k]; Fi.x += g * t.x; Fi.y -= g * t.y; }
+                    if (k != j0) { const float g = rj[k]; Fj.x += g * t.x; Fj.y -= g * t.y; }
+                }
+            }
+            const int oi = a.site_off[i0], ni_ = a.site_off[i0 + 1] - oi;
+            const int oj = a.site_off[j0], nj_ = a.site_off[j0 + 1] - oj;
+            for (int e = tid; e < ni_; e += bdim) {
+                const uint2 en = a.site_ent[oi + e];
+                const float2 tp = s32[en.x & 0xff], tm = s32[(en.x >> 8) & 0xff], tn = s32[(en.x >> 16) & 0xff];
+                const float g = __uint_as_float(en.y);
+                const float2 mn = cmulf(tm, tn);
+                // X = tp * conj(tm tn)
+                Fi.x += g * (tp.x * mn.x + tp.y * mn.y);
+                Fi.y += g * (tp.y * mn.x - tp.x * mn.y);
+            }
+            for (int e = tid; e < nj_; e += bdim) {
+                const uint2 en = a.site_ent[oj + e];
+                const float2 tp = s32[en.x & 0xff], tm = s32[(en.x >> 8) & 0xff], tn = s32[(en.x >> 16) & 0xff];
+                const float g = __uint_as_float(en.y);
+                const float2 mn = cmulf(tm, tn);
+                Fj.x += g * (tp.x * mn.x + tp.y * mn.y);
+                Fj.y += g * (tp.y * mn.x - tp.x * mn.y);
+            }
+            float local = -(di.x * Fi.x - di.y * Fi.y) - (dj.x * Fj.x - dj.y * Fj.y);
+            local = wave_sum(local);
+            if (lane == 0) part[step & 1][w] = local;
+            __syncthreads();
+        }
+    }
+
+    for (int i = tid; i < N; i += bdim) {
+        gs[2 * i] = s64[i].x;
+        gs[2 * i + 1] = s64[i].y;
+    }
+    if (tid == 0) {
+        a.E[rep] = E;
+        a.acc[rep] += nacc;
+        a.prop[rep] += nprop;
+    }
+}
+
+hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
+    SweepArgs a;
+    a.N = st->N;
+    a.nrep = st->nrep;
+    a.h2_active = st->h2_active;
+    a.n_sweeps = n_sweeps;
+    a.spins = st->d_spins;
+    a.g2 = st->d_g2_32;
+    a.site_off = st->d_site_off;
+    a.site_ent = st->d_site_ent;
+    a.betas = st->d_betas;
+    a.E = st->d_E;
+    a.acc = st->d_acc;
+    a.prop = st->d_prop;
+    a.seeds = st->d_seeds;
+    a.sweep0 = st->sweep_counter;
+    a.log = st->d_log;
+    a.log_cap = st->log_cap;
+    a.log_base = st->log_n;
+    int bs = st->block_size;
+    if (bs <= 0) {
+        // Enough threads that each touches a few entries per step.
+        long long per_step = 2LL * st->max_site_len;
+        bs = per_step >= 8192 ? 1024 : per_step >= 1024 ? 256 : 64;
+    }
+    if (st->variant == PSPIN_SPARSE)
+        hipLaunchKernelGGL(k_sweep_lists<true>, dim3(st->nrep), dim3(bs), 0, st->stream, a);
+    else
+        hipLaunchKernelGGL(k_sweep_lists<false>, dim3(st->nrep), dim3(bs), 0, st->stream, a);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        st->sweep_counter += (uint64_t)n_sweeps;
+        if (st->d_log) st->log_n = std::min(st->log_cap, st->log_n + n_sweeps);
+    }
+    return e;
+}
+
+}  // namespace ps

@@ -1,0 +1,198 @@
+/* pspin.h -- C ABI of the MI355X-native 2+4 p-spin Monte Carlo library
+ * (libpspin.so, hand-written gfx950 HIP kernels).
+ *
+ * The reference exposes a C++ header API (no FFI): MCState / MCStateSparse
+ * plus free functions.  Every entry point below replaces one of them; the
+ * citation names the reference declaration (file:line).  A C++ mirror with
+ * the reference's own names lives in include/pspin_mc.hpp.
+ *
+ * Conventions
+ *  - Every call returns PSPIN_OK (0) or a negative error code; the message is
+ *    available from pspin_last_error().  (The reference prints and exit(1)s
+ *    on any CUDA error, src/mc.cu:12-20; the C++ mirror restores that.)
+ *  - Calls are synchronous on return, like the reference's API, which ends
+ *    every call with cudaDeviceSynchronize (src/mc.cu:765,799).
+ *  - One handle is single-threaded.  Host buffers are caller-allocated.
+ *  - Spins cross the boundary as interleaved (re, im) doubles, replica-major
+ *    [nrep][N][2], the byte layout of cuDoubleComplex arrays in the reference.
+ */
+#ifndef PSPIN_H
+#define PSPIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSPIN_OK 0
+#define PSPIN_ERR_ARG (-1)
+#define PSPIN_ERR_HIP (-2)
+#define PSPIN_ERR_STATE (-3)
+#define PSPIN_ERR_NOMEM (-4)
+
+/* Constraint / disorder variants. */
+#define PSPIN_DENSE 0   /* spherical sum|a|^2=N, dense C(N,4) H4 (mc.h)            */
+#define PSPIN_SPARSE 1  /* smoothed cube sum|a|^4=N, N sampled quartets (mc_sparse.h) */
+
+/* Same fields and meaning as SimConfig (include/config.h:9-25). */
+typedef struct pspin_sim_config {
+    int save_freq;
+    int verbose;
+    int N;
+    int nrep;
+    double T;
+    double J;
+    double J0;
+    double alpha;
+    double alpha0;
+    int mc_iterations;
+    uint64_t seed;
+    int label;
+    int dev;
+    int fmc_mode; /* 0 fully connected, 1 comb, 2 uniform */
+    double gamma;
+} pspin_sim_config;
+
+typedef struct pspin_info {
+    int N;
+    int nrep;
+    int variant;
+    int h2_active;
+    long long n_pairs_active;  /* MCState::n_pairs_active (include/mc.h:24) */
+    long long n_quart_active;  /* MCState::n_quart_active (include/mc.h:25) */
+    int n_quartets;            /* MCStateSparse::n_quartets (include/mc_sparse.h:19); C(N,4) if dense */
+    long long n_site_entries;  /* stored (site, quartet) H4 entries used by the sweep */
+    int coupling_format;       /* PSPIN_COUPLING_* */
+    int kernel;                /* PSPIN_KERNEL_* in use */
+} pspin_info;
+
+typedef struct pspin_state pspin_state;
+
+const char* pspin_last_error(void);
+
+/* Default SimConfig values (include/config.h:28-46). */
+void pspin_default_config(pspin_sim_config* cfg);
+
+/* parse_args (src/config.cu:6-70): same flags, same auto-gamma.  Returns
+ * PSPIN_ERR_ARG on an unknown flag or N<4 (the reference exits 1). */
+int pspin_parse_args(int argc, char** argv, pspin_sim_config* cfg);
+
+/* mc_init (include/mc.h:33) / mc_sparse_init (include/mc_sparse.h:42):
+ * disorder (g2, g4, FMC mask), spins, RNG keys and FP64 initial energies.
+ * variant: PSPIN_DENSE or PSPIN_SPARSE. */
+int pspin_create(const pspin_sim_config* cfg, int variant, pspin_state** out);
+
+/* mc_free (include/mc.h:36) / mc_sparse_free (include/mc_sparse.h:45). */
+int pspin_destroy(pspin_state* st);
+
+int pspin_get_info(const pspin_state* st, pspin_info* info);
+
+/* mc_set_betas (include/mc.h:52) / mc_sparse_set_betas (mc_sparse.h:51):
+ * betas[nrep] by physical replica. */
+int pspin_set_betas(pspin_state* st, const double* betas);
+
+/* mc_sweep_pt (include/mc.h:49) / mc_sparse_sweep_pt (mc_sparse.h:48),
+ * n_sweeps times with the current betas.  mc_sweep (include/mc.h:39) is
+ * pspin_set_uniform_beta(1/T) followed by this call. */
+int pspin_sweep(pspin_state* st, int n_sweeps);
+int pspin_set_uniform_beta(pspin_state* st, double beta);
+
+/* mc_get_results (include/mc.h:42-43). */
+int pspin_get_results(const pspin_state* st, double* E, long long* accepted, long long* proposed);
+
+/* The drivers' cudaMemset of d_accepted / d_proposed
+ * (src/parallel_tempering.cu:396-397, src/simulated_annealing.cu:205-206). */
+int pspin_reset_counters(pspin_state* st);
+
+/* mc_get_spins (include/mc.h:46) / mc_sparse_get_spins (mc_sparse.h:57). */
+int pspin_get_spins(const pspin_state* st, double* re_im);
+
+/* Direct writes of d_spins in the reference tests (src/test_delta_e.cu:138);
+ * recomputes the FP64 energies. */
+int pspin_set_spins(pspin_state* st, const double* re_im);
+
+/* mc_compute_split_energies (include/mc.h:56) / sparse (mc_sparse.h:60):
+ * FP64 E2, E4 per replica, i<j H2 convention. */
+int pspin_split_energies(const pspin_state* st, double* e2, double* e4);
+
+/* compute_energy (include/hamiltonian.h:11-17) for all replicas, written
+ * into the running energies: the FP64 resync of the running accumulator. */
+int pspin_resync_energy(pspin_state* st);
+int pspin_set_energies(pspin_state* st, const double* E); /* test hook */
+
+/* Disorder read-back, for tests and the drivers' output files
+ * (frequencies.txt src/parallel_tempering.cu:249-256; quartets.txt
+ * src/parallel_tempering_sparse.cu:277-296; d_g2 / d_g4 / d_g4_mask reads in
+ * src/test_delta_e.cu:138-145).  omega[N] (zeros in FC mode), g2[N*N],
+ * g4[C(N,4)] and mask[C(N,4)] (dense only), quartets[n_quartets*5] as
+ * (i,j,k,l,ch) and gq[n_quartets] (sparse only). */
+int pspin_get_omega(const pspin_state* st, double* omega);
+int pspin_get_g2(const pspin_state* st, double* g2);
+int pspin_get_g4(const pspin_state* st, double* g4, uint8_t* mask);
+int pspin_get_sparse_quartets(const pspin_state* st, int* quartets, double* gq);
+
+/* ---------------- parallel tempering (src/parallel_tempering.cu) -------- */
+/* Set up NT x nrep_per_temp replicas (nrep must equal their product),
+ * betas from beta_sched[NT] (logical index t*nrep_per_temp+r, :273-278), the
+ * identity permutation (:283-284) and the exchange stream srand48(seed48)
+ * (:304; pass (long)seed + 5000 to match the reference). */
+int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta_sched, long seed48);
+
+/* The driver's main loop (:312-352) on the GPU: n_sweeps sweeps, with a
+ * replica exchange after every sweep s where (sweep_count+1) % pt_freq == 0.
+ * Exchange decisions follow the reference exactly: hot-to-cold sequential
+ * order, delta = (b_t - b_{t+1})(E_a - E_b), one drand48 draw per delta<0. */
+int pspin_pt_run(pspin_state* st, int n_sweeps, int pt_freq);
+
+/* One exchange round using the current running energies (test hook). */
+int pspin_pt_exchange(pspin_state* st);
+
+/* perm[NT*nrep_per_temp]: logical -> physical replica (:280-284). */
+int pspin_pt_get_perm(const pspin_state* st, int* perm);
+
+/* Exchange counters per temperature pair [NT-1]: window (since the last
+ * pspin_pt_reset_window) and cumulative (:297-301). */
+int pspin_pt_get_exchange_stats(const pspin_state* st, long long* acc_win, long long* prop_win,
+                                long long* acc_tot, long long* prop_tot);
+int pspin_pt_reset_window(pspin_state* st);
+
+/* drand48 state of the exchange stream (48 bits), for checking against libc. */
+int pspin_pt_get_rng48(const pspin_state* st, uint64_t* x48);
+
+/* Energy at the coldest slot (t = NT-1) of each ladder r, recorded after
+ * every exchange round into a device ring of `capacity` rounds (0 disables).
+ * pspin_pt_get_series copies min(n_recorded, capacity) rounds oldest first,
+ * layout [round][nrep_per_temp]; *n_out receives the number of rounds. */
+int pspin_pt_enable_series(pspin_state* st, int capacity);
+int pspin_pt_get_series(const pspin_state* st, double* out, int* n_out);
+
+/* ---------------- test and profiling hooks (no reference counterpart) --- */
+/* Per-step log of the next sweeps: capacity_sweeps sweeps per replica.
+ * Record layout (48 bytes): int i0, j0, accepted; float dE32;
+ * double ai_new[2], aj_new[2].  Layout [rep][sweep][step]. */
+int pspin_enable_step_log(pspin_state* st, int capacity_sweeps);
+int pspin_get_step_log(const pspin_state* st, void* out, int* n_sweeps_logged);
+
+/* Per-replica RNG seeds (splitmix64 chain, src/mc.cu:418-423) and the global
+ * sweep counter that keys each sweep's Philox stream. */
+int pspin_get_rng_info(const pspin_state* st, uint64_t* seeds, uint64_t* sweep_counter);
+
+/* Sweep-kernel timing: when on, every sweep launch is bracketed by HIP
+ * events on the library's stream; totals accumulate until reset. */
+int pspin_set_profiling(pspin_state* st, int on);
+int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweep_launches, double* exch_ms,
+                      long long* n_exch_launches);
+
+/* Kernel selection and coupling storage (for the autotuner and tests). */
+#define PSPIN_KERNEL_AUTO 0
+#define PSPIN_KERNEL_LISTS 1  /* per-site (partner, conj, conj, g) lists       */
+#define PSPIN_KERNEL_PAIRS 2  /* dense FC per-site pair-row layout             */
+#define PSPIN_COUPLING_F32 0
+#define PSPIN_COUPLING_BF16X2 1 /* BF16 hi/lo split, FP32 accumulation       */
+int pspin_set_kernel(pspin_state* st, int kernel, int block_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSPIN_H */

@@ -1,0 +1,90 @@
+"""GPU parallel tempering (libpspin.so) vs the reference's host exchange.
+
+Bit-exact: given the same energies and srand48 seed, the GPU exchange makes
+the same decisions as src/parallel_tempering.cu:316-352 run with glibc
+drand48 (the oracle), leaves the same permutation, counters and betas, and
+the same 48-bit stream state.  Then the reference's test_replica_exchange
+(src/test_replica_exchange.cu "8 6 65536 5", rates in [0,1] hard, soft
+monotonicity/ergodicity reported), and the T_min energy series.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("NT,nrep", [(6, 1), (32, 8), (5, 100), (3, 64)])
+def test_exchange_bit_exact(pspin, oracle, NT, nrep):
+    print(f"[exchange bit-exact] NT={NT} nrep={nrep}", flush=True)
+    cfg = pspin.SimConfig.default(N=8, nrep=NT * nrep, seed=5)
+    st = pspin.PSpin(cfg, 0)
+    T, beta = oracle.temperatures(NT, 0.2, 1.6, 1)
+    seed48 = 42 + 5000
+    st.pt_init(NT, nrep, beta, seed48)
+    oracle.srand48(seed48)
+    perm = np.arange(NT * nrep, dtype=np.int32)
+    acc = np.zeros(NT - 1, dtype=np.int64)
+    prop = np.zeros(NT - 1, dtype=np.int64)
+    rng = np.random.default_rng(NT * 1000 + nrep)
+    for rnd in range(25):
+        # energies with a trend along the ladder plus noise, so both branches occur
+        E = rng.normal(size=NT * nrep) * 3.0 + np.linspace(0, -5, NT * nrep)
+        st.set_energies(E)
+        st.pt_exchange()
+        betas = oracle.pt_exchange(NT, nrep, beta, E, perm, acc, prop)
+        gperm = st.pt_perm()
+        assert np.array_equal(gperm, perm), f"round {rnd}"
+    acc_w, prop_w, acc_t, prop_t = st.pt_exchange_stats()
+    assert np.array_equal(acc_t, acc) and np.array_equal(prop_t, prop)
+    assert np.array_equal(acc_w, acc) and np.array_equal(prop_w, prop)
+    # stream state: advance glibc by one more draw on both sides and compare
+    x = st.pt_rng48()
+    nxt = (0x5DEECE66D * x + 0xB) & ((1 << 48) - 1)
+    assert oracle.drand48() == nxt / 2.0 ** 48
+    st.pt_reset_window()
+    acc_w, prop_w, acc_t2, _ = st.pt_exchange_stats()
+    assert not acc_w.any() and not prop_w.any() and np.array_equal(acc_t2, acc)
+    st.close()
+
+
+def test_reference_replica_exchange(pspin, oracle):
+    # src/test_replica_exchange.cu with run_tests.sh "8 6 65536 5" (seed 65432,
+    # linear T in [0.5, 2], srand48(seed+7777)); 16384 sweeps here.
+    N, NT, sweeps, pt_freq = 8, 6, 16384, 5
+    cfg = pspin.SimConfig.default(N=N, nrep=NT, T=2.0, J=2.0, J0=0.0, alpha=0.5, alpha0=0.0, seed=65432)
+    st = pspin.PSpin(cfg, 0)
+    T, beta = oracle.temperatures(NT, 0.5, 2.0, 0)
+    st.pt_init(NT, 1, beta, 65432 + 7777)
+    st.pt_enable_series(sweeps // pt_freq + 8)
+    st.pt_run(sweeps, pt_freq)
+    _, _, acc, prop = st.pt_exchange_stats()
+    assert np.all(prop == sweeps // pt_freq)
+    rates = acc / prop
+    assert np.all((rates >= 0) & (rates <= 1))
+    print("  rates:", np.round(rates, 4))
+    viol = int(np.sum(rates[1:] > rates[:-1] + 0.15))
+    print(f"  monotonicity violations (soft): {viol}")
+    # T_min series = energy of the replica at the coldest slot after each round
+    ser = st.pt_series()
+    assert ser.shape == (sweeps // pt_freq, 1)
+    E, _, _ = st.results()
+    perm = st.pt_perm()
+    assert ser[-1, 0] == E[perm[NT - 1]]
+    st.close()
+
+
+def test_pt_run_counts_and_betas(pspin, oracle):
+    NT, nr = 4, 3
+    cfg = pspin.SimConfig.default(N=12, nrep=NT * nr, seed=17)
+    st = pspin.PSpin(cfg, 0)
+    T, beta = oracle.temperatures(NT, 0.3, 1.5, 1)
+    st.pt_init(NT, nr, beta, 17 + 5000)
+    st.pt_run(7, 3)   # exchanges after sweeps 3 and 6
+    st.pt_run(5, 3)   # after 9 and 12
+    _, _, acc, prop = st.pt_exchange_stats()
+    assert np.all(prop == 4 * nr)
+    _, _, p = st.results()
+    assert np.all(p == 12 * (12 // 2))
+    perm = st.pt_perm()
+    assert sorted(perm.tolist()) == list(range(NT * nr))
+    st.close()

@@ -1,0 +1,256 @@
+"""GPU parity of the sweep kernel (libpspin.so) against the FP64 oracle.
+
+Per-proposal parity: the kernel logs every step (pair, proposed amplitudes,
+FP32 Delta E, decision).  The oracle regenerates the step's uniforms from the
+same Philox stream, re-derives pair and proposal (FP64), recomputes Delta E
+in FP64 by the reference's 3-type enumeration (src/mc.cu:160-275) and
+replays the chain following the GPU's decisions.
+
+Tolerance (FP32 Delta E): |dE32 - dE64| <= 1e-5 * S, S = sum over touched
+terms of |g| (|f_old| + |f_new|); the largest ratio seen is also reported
+relative to |dE64|.
+
+The reference's own integration tests follow at their run_tests.sh arguments
+(src/test_delta_e.cu, test_spherical.cu, test_inf_temp.cu,
+test_detailed_balance.cu, test_sparse_dense.cu).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DE_TOL = 1e-5
+
+
+def _system(st, O):
+    g2 = st.g2()
+    if st.variant == 0:
+        g4, mask = st.g4()
+        return dict(g2=g2, g4=g4, mask=mask)
+    q, gq = st.sparse_quartets()
+    return dict(g2=g2, q=q, gq=gq)
+
+
+def _dE(O, st, sysd, s, i0, j0, ni, nj):
+    if st.variant == 0:
+        return O.delta_e(st.N, s, sysd["g2"], st.info.h2_active, sysd["g4"], sysd["mask"], i0, j0, ni, nj)
+    return O.delta_e_sparse(st.N, s, sysd["g2"], st.info.h2_active, sysd["q"], sysd["gq"], i0, j0, ni, nj)
+
+
+def _energy(O, st, sysd, s):
+    if st.variant == 0:
+        return sum(O.energy(st.N, s, sysd["g2"], st.info.h2_active, sysd["g4"], sysd["mask"]))
+    return sum(O.energy_sparse(st.N, s, sysd["g2"], st.info.h2_active, sysd["q"], sysd["gq"]))
+
+
+def replay_check(O, st, n_sweeps, betas):
+    """Run n_sweeps logged sweeps and replay them on the oracle. Returns stats."""
+    N, nrep, steps = st.N, st.nrep, st.N // 2
+    sysd = _system(st, O)
+    s0 = st.spins()
+    E0, _, _ = st.results()
+    seeds, sweep0 = st.rng_info()
+    st.set_betas(betas)
+    st.enable_step_log(n_sweeps)
+    st.sweep(n_sweeps)
+    log, n = st.step_log()
+    assert n == n_sweeps
+    s_gpu = st.spins()
+    E_gpu, acc, prop = st.results()
+    worst = 0.0
+    worst_rel = 0.0
+    n_acc = 0
+    for r in range(nrep):
+        s = s0[r].copy()
+        E = float(E0[r])
+        beta = float(betas[r])
+        for sw in range(n_sweeps):
+            for k in range(steps):
+                rec = log[r, sw, k]
+                u = O.step_uniforms(int(seeds[r]), sweep0 + sw, k)
+                i0, j0 = O.pair_pick(N, u[0], u[1])
+                assert (int(rec["i0"]), int(rec["j0"])) == (i0, j0), (r, sw, k)
+                if st.variant == 0:
+                    ni, nj = O.propose_sphere(s[i0], s[j0], u[2], u[3], u[4])
+                else:
+                    ni, nj = O.propose_cube(s[i0], s[j0], u[2], u[3], u[4])
+                np.testing.assert_allclose(rec["ai"], ni, rtol=0, atol=1e-12)
+                np.testing.assert_allclose(rec["aj"], nj, rtol=0, atol=1e-12)
+                dE64, scale = _dE(O, st, sysd, s, i0, j0, rec["ai"], rec["aj"])
+                err = abs(float(rec["dE"]) - dE64)
+                assert err <= DE_TOL * scale + 1e-30, (r, sw, k, float(rec["dE"]), dE64, scale)
+                worst = max(worst, err / max(scale, 1e-300))
+                worst_rel = max(worst_rel, err / max(abs(dE64), 1e-3 * scale, 1e-300))
+                thr = math.inf if beta == 0.0 else -math.log(u[5]) / beta
+                want = beta == 0.0 or float(rec["dE"]) < thr
+                if abs(float(rec["dE"]) - thr) > 1e-9 * (1.0 + abs(thr)):
+                    assert bool(rec["acc"]) == want, (r, sw, k)
+                if rec["acc"]:
+                    s[i0] = complex(*rec["ai"])
+                    s[j0] = complex(*rec["aj"])
+                    E += float(np.float32(rec["dE"]))
+                    n_acc += 1
+        np.testing.assert_array_equal(s_gpu[r], s)
+        assert abs(E_gpu[r] - E) <= 1e-9 * max(1.0, abs(E))
+        E_full = _energy(O, st, sysd, s)
+        assert abs(E_gpu[r] - E_full) / N <= 1e-5, (E_gpu[r], E_full)
+    assert int(prop.sum()) == nrep * n_sweeps * steps
+    assert int(acc.sum()) == n_acc
+    return dict(worst_scaled=worst, worst_rel=worst_rel, acc_rate=n_acc / (nrep * n_sweeps * steps))
+
+
+@pytest.mark.parametrize("name,variant,kw,T,nsw", [
+    ("dense FC N=12", 0, dict(N=12, fmc_mode=0), 1.0, 6),
+    ("dense uniform N=32 (config-1 system)", 0, dict(N=32, fmc_mode=2, gamma=0.05), 0.4, 3),
+    ("dense comb N=16", 0, dict(N=16, fmc_mode=1, gamma=1.0), 0.8, 4),
+    ("dense FC N=20 J0", 0, dict(N=20, fmc_mode=0, J0=0.4, alpha0=0.5), 0.6, 2),
+    ("sparse cube N=16", 1, dict(N=16, fmc_mode=2, gamma=0.05), 0.5, 10),
+    ("sparse cube FC N=24", 1, dict(N=24, fmc_mode=0), 1.0, 6),
+])
+def test_step_replay(pspin, oracle, name, variant, kw, T, nsw):
+    print(f"[step replay] {name}", flush=True)
+    cfg = pspin.SimConfig.default(nrep=4, seed=1234, **kw)
+    st = pspin.PSpin(cfg, variant)
+    betas = np.array([1.0 / T, 2.0 / T, 0.5 / T, 0.0])
+    stats = replay_check(oracle, st, nsw, betas)
+    print(f"  max |dE32-dE64|/S = {stats['worst_scaled']:.2e}  vs |dE64|: {stats['worst_rel']:.2e}  "
+          f"acc={stats['acc_rate']:.3f}", flush=True)
+    st.close()
+
+
+def test_block_sizes_agree(pspin, oracle):
+    # Same chain at different workgroup sizes: identical pairs and proposals,
+    # Delta E within tolerance (different reduction order), decisions equal.
+    res = []
+    for bs in (64, 256, 1024):
+        cfg = pspin.SimConfig.default(N=24, nrep=2, seed=99, fmc_mode=0)
+        st = pspin.PSpin(cfg, 0)
+        st.set_kernel(1, bs)
+        st.set_uniform_beta(1.5)
+        st.sweep(20)
+        res.append((st.spins(), st.results()[0]))
+        st.close()
+    for s, E in res[1:]:
+        np.testing.assert_allclose(s, res[0][0], atol=1e-9)
+        np.testing.assert_allclose(E, res[0][1], rtol=1e-6)
+
+
+def test_reference_delta_e(pspin, oracle):
+    # src/test_delta_e.cu with run_tests.sh args "10 4 200": FC, seed 77777,
+    # |E_stored - E_full|/N <= 1e-4 at 10 checkpoints (E_full from the FP64
+    # oracle, i<j convention).
+    N, nrep, sweeps = 10, 4, 200
+    cfg = pspin.SimConfig.default(N=N, nrep=nrep, T=1.0, J=2.0, J0=0.0, alpha=0.5, alpha0=0.0, seed=77777, fmc_mode=0)
+    st = pspin.PSpin(cfg, 0)
+    sysd = _system(st, oracle)
+    st.set_uniform_beta(1.0)
+    worst = 0.0
+    for chk in range(10):
+        st.sweep(sweeps // 10)
+        E, _, _ = st.results()
+        s = st.spins()
+        for r in range(nrep):
+            d = abs(E[r] - _energy(oracle, st, sysd, s[r])) / N
+            worst = max(worst, d)
+            assert d <= 1e-4
+    print(f"  max |E_stored-E_full|/N = {worst:.2e}")
+    st.close()
+
+
+def test_reference_spherical(pspin):
+    # src/test_spherical.cu "16 4 500": sum|a|^2 = N within 1e-8.
+    cfg = pspin.SimConfig.default(N=16, nrep=4, seed=555, fmc_mode=0)
+    st = pspin.PSpin(cfg, 0)
+    st.set_uniform_beta(1.0)
+    assert np.max(np.abs(np.sum(np.abs(st.spins()) ** 2, axis=1) - 16)) <= 1e-8
+    st.sweep(500)
+    assert np.max(np.abs(np.sum(np.abs(st.spins()) ** 2, axis=1) - 16)) <= 1e-8
+    st.close()
+
+
+def test_reference_inf_temp(pspin):
+    # src/test_inf_temp.cu "12 8 500": T = 1e30 -> acceptance >= 0.999, |<E/N>| < 5.
+    cfg = pspin.SimConfig.default(N=12, nrep=8, seed=31337, fmc_mode=0, T=1e30)
+    st = pspin.PSpin(cfg, 0)
+    st.set_uniform_beta(1.0 / 1e30)
+    es = []
+    for _ in range(50):
+        st.sweep(10)
+        es.append(st.results()[0] / 12)
+    E, acc, prop = st.results()
+    assert np.min(acc / prop) >= 0.999
+    assert abs(np.mean(es)) < 5
+    st.close()
+
+
+def test_reference_detailed_balance(pspin):
+    # src/test_detailed_balance.cu "8 8 500 1000": two disorder samples at
+    # T=0.8 agree within 5 combined standard errors; both variances > 0.
+    N, nrep, equil, measure, T = 8, 8, 500, 1000, 0.8
+    sts = []
+    for seed in (11111, 99999):
+        cfg = pspin.SimConfig.default(N=N, nrep=nrep, T=T, J=2.0, J0=0.0, alpha=0.5, alpha0=0.0, seed=seed)
+        st = pspin.PSpin(cfg, 0)
+        st.set_uniform_beta(1.0 / T)
+        st.sweep(equil)
+        sts.append(st)
+    samples = [[], []]
+    for _ in range(measure // 5):
+        for g, st in enumerate(sts):
+            st.sweep(5)
+            samples[g].extend((st.results()[0] / N).tolist())
+    a, b = np.array(samples[0]), np.array(samples[1])
+    se = math.sqrt(a.var() / len(a) + b.var() / len(b))
+    assert abs(a.mean() - b.mean()) / se < 5.0
+    assert a.var() > 0 and b.var() > 0
+    for st in sts:
+        st.close()
+
+
+def test_reference_sparse_dense(pspin, oracle):
+    # src/test_sparse_dense.cu "10 4 300": cube constraint within 1e-8, running E
+    # vs the CPU sparse oracle within 1e-3*N, T=1e30 acceptance >= 0.99.
+    N, nrep, sweeps = 10, 4, 300
+    cfg = pspin.SimConfig.default(N=N, nrep=nrep, T=1.0, J=2.0, J0=0.0, alpha=0.5, alpha0=0.0, seed=54321)
+    st = pspin.PSpin(cfg, 1)
+    sysd = _system(st, oracle)
+    st.set_uniform_beta(1.0)
+    assert np.max(np.abs(np.sum(np.abs(st.spins()) ** 4, axis=1) - N)) <= 1e-8
+    for _ in range(10):
+        st.sweep(sweeps // 10)
+        s = st.spins()
+        E, _, _ = st.results()
+        assert np.max(np.abs(np.sum(np.abs(s) ** 4, axis=1) - N)) <= 1e-8
+        for r in range(nrep):
+            assert abs(E[r] - _energy(oracle, st, sysd, s[r])) / N <= 1e-3
+    st.close()
+    cfg = pspin.SimConfig.default(N=N, nrep=nrep, T=1e30, seed=98765)
+    st = pspin.PSpin(cfg, 1)
+    st.set_uniform_beta(1e-30)
+    st.sweep(200)
+    E, acc, prop = st.results()
+    assert np.min(acc / prop) >= 0.99
+    st.close()
+
+
+def test_beta0_stationary_laws(pspin):
+    # At beta = 0 the spherical pair rotation samples intensities
+    # I_k = |a_k|^2/N ~ Beta(1/2, (N-1)/2): E[I^2] = 3/(N(N+2)); the cube move
+    # samples s_k = |a_k|^4/N ~ Dirichlet(1): E[s^2] = 2/(N(N+1)).
+    N, nrep = 8, 256
+    for variant, want in ((0, 3.0 / (N * (N + 2))), (1, 2.0 / (N * (N + 1)))):
+        cfg = pspin.SimConfig.default(N=N, nrep=nrep, seed=7 + variant)
+        st = pspin.PSpin(cfg, variant)
+        st.set_uniform_beta(0.0)
+        st.sweep(50)
+        acc2 = []
+        for _ in range(40):
+            st.sweep(5)
+            s = st.spins()
+            x = (np.abs(s) ** 2 / N) if variant == 0 else (np.abs(s) ** 4 / N)
+            acc2.append(np.mean(x ** 2))
+        m = float(np.mean(acc2))
+        assert abs(m - want) / want < 0.03, (variant, m, want)
+        st.close()
