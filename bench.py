@@ -98,6 +98,38 @@ def cpu_baseline(st, N, nrep, betas, seed, n_sweeps):
                       f"({pu} pair-updates, {secs:.1f} s, FP64, reference 3-type enumeration)"}
 
 
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of known size (hipMemcpy, HIP's own blit
+    kernel), run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to calibrate
+    the counters against bytes actually moved."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(4):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 4}))
+
+
+def load_traffic(N, fmc, kernel):
+    """HBM bytes per sweep launch measured with rocprofv3 --pmc (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        rec = json.load(open(path))
+    except ValueError:
+        return None
+    key = f"N{N}_fmc{fmc}_k{kernel}"
+    return rec.get(key, {}).get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,7 +145,11 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--cpu-sweeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, help="only run the counter-calibration copy of this size")
     args = ap.parse_args()
+    if args.copy_calib:
+        copy_calibration(args.copy_calib)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(independent ladders would shard with no communication). Run with --gpus 1.", file=sys.stderr)
@@ -172,7 +208,7 @@ def main():
         "exchange_kernel_ms": prof["exch_ms"] / max(prof["n_exch"], 1),
         "init_s": t_init,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": load_traffic(N, args.fmc, st.info.kernel),
                      "bytes_per_pair_update": bpu, "pair_updates_per_launch": pu_per_step},
     }
     if not args.no_cpu:

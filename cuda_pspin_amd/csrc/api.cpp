@@ -4,6 +4,7 @@
 // src/mc_sparse.cu:390-604) and of its PT driver loop
 // (src/parallel_tempering.cu:273-352).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -69,6 +70,8 @@ void free_all(pspin_state* st) {
     dfree(st->d_seeds); dfree(st->d_e2); dfree(st->d_e4);
     dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48);
     dfree(st->d_series); dfree(st->d_series_n); dfree(st->d_log);
+    dfree(st->d_ptab); dfree(st->d_row_off); dfree(st->d_row_len); dfree(st->d_pair_bc);
+    dfree(st->d_tasks); dfree(st->d_wave_tasks);
     for (auto& pr : st->ev_sweep) { st->ev_free.push_back(pr.first); st->ev_free.push_back(pr.second); }
     for (auto& pr : st->ev_exch) { st->ev_free.push_back(pr.first); st->ev_free.push_back(pr.second); }
     st->ev_sweep.clear(); st->ev_exch.clear();
@@ -126,6 +129,87 @@ int build_site_lists(pspin_state* st, const std::vector<Term>& terms) {
     PS_TRY_HIP(cpy(st, st->d_site_off, cnt.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
     if (!ent.empty())
         PS_TRY_HIP(cpy(st, st->d_site_ent, ent.data(), ent.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    return PSPIN_OK;
+}
+
+// Dense per-site pair-row slices for the fully connected case (every quartet
+// channel 0).  Others of site x are relabelled 0..M-1 (M = N-1, o(y) = y + (y >= x)).
+// Pairs (b<c) are ordered by b descending, so the pairs with b > a -- row a --
+// are a prefix of length C(M-1-a, 2).  Rows are padded to a multiple of 4
+// floats with zeros.  The task list splits each 64-lane block of 4-pair chunks
+// into row ranges of at most kRows rows and deals them longest-first to the
+// waves of the workgroup.
+int build_pair_layout(pspin_state* st, const std::vector<double>& g4, const std::vector<uint8_t>& mask) {
+    const int N = st->N, M = N - 1;
+    const int nrows = M - 2;
+    std::vector<uint32_t> pbc;
+    for (int b = M - 2; b >= 1; b--)
+        for (int c = b + 1; c < M; c++) pbc.push_back((uint32_t)b | ((uint32_t)c << 16));
+    const int npairs = (int)pbc.size();
+    std::vector<int> roff(nrows), rlen(nrows);
+    long long off = 0;
+    for (int a = 0; a < nrows; a++) {
+        long long L = (long long)(M - 1 - a) * (M - 2 - a) / 2;
+        roff[a] = (int)off;
+        rlen[a] = (int)((L + 3) / 4 * 4);
+        off += rlen[a];
+    }
+    const int slice = (int)((off + 15) / 16 * 16);
+    std::vector<float> tab((size_t)N * slice, 0.0f);
+    for (int x = 0; x < N; x++) {
+        float* sl = &tab[(size_t)x * slice];
+        for (int a = 0; a < nrows; a++) {
+            const long long L = (long long)(M - 1 - a) * (M - 2 - a) / 2;
+            for (long long P = 0; P < L; P++) {
+                const int b = pbc[P] & 0xffff, c = pbc[P] >> 16;
+                int v[4] = {x, a + (a >= x), b + (b >= x), c + (c >= x)};
+                std::sort(v, v + 4);
+                const long long q = (long long)v[3] * (v[3] - 1) * (v[3] - 2) * (v[3] - 3) / 24 +
+                                    (long long)v[2] * (v[2] - 1) * (v[2] - 2) / 6 + (long long)v[1] * (v[1] - 1) / 2 + v[0];
+                sl[roff[a] + P] = (mask[q] == 1u) ? (float)g4[q] : 0.0f;
+            }
+        }
+    }
+    // tasks: blocks of 64 chunks of 4 pairs; rows [0, b_max(block))
+    const int nchunks = (npairs + 3) / 4, nblocks = (nchunks + 63) / 64;
+    constexpr int kRows = 6;
+    std::vector<std::array<int, 4>> tasks;
+    for (int slot = 0; slot < 2; slot++)
+        for (int blk = 0; blk < nblocks; blk++) {
+            const int bmax = (int)(pbc[(size_t)blk * 256] & 0xffff);  // rows a < bmax
+            for (int a0 = 0; a0 < bmax; a0 += kRows) tasks.push_back({slot, blk * 64, a0, std::min(bmax, a0 + kRows)});
+        }
+    const int nw = std::max(1, std::min<int>(16, (int)tasks.size()));
+    std::stable_sort(tasks.begin(), tasks.end(),
+                     [](const std::array<int, 4>& p, const std::array<int, 4>& q) { return p[3] - p[2] > q[3] - q[2]; });
+    std::vector<std::vector<std::array<int, 4>>> per(nw);
+    std::vector<int> load(nw, 0);
+    for (const auto& t : tasks) {
+        int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        per[w].push_back(t);
+        load[w] += (t[3] - t[2]) + 2;  // rows plus per-task set-up
+    }
+    std::vector<int4> flat;
+    std::vector<int> wt(nw + 1, 0);
+    for (int w = 0; w < nw; w++) {
+        for (const auto& t : per[w]) flat.push_back(make_int4(t[0], t[1], t[2], t[3]));
+        wt[w + 1] = (int)flat.size();
+    }
+    st->pair_slice_len = slice;
+    st->pair_npairs = npairs;
+    st->pair_nwaves = nw;
+    PS_TRY_HIP(dmalloc(&st->d_ptab, tab.size()));
+    PS_TRY_HIP(dmalloc(&st->d_row_off, nrows));
+    PS_TRY_HIP(dmalloc(&st->d_row_len, nrows));
+    PS_TRY_HIP(dmalloc(&st->d_pair_bc, pbc.size()));
+    PS_TRY_HIP(dmalloc(&st->d_tasks, flat.size()));
+    PS_TRY_HIP(dmalloc(&st->d_wave_tasks, wt.size()));
+    PS_TRY_HIP(cpy(st, st->d_ptab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    PS_TRY_HIP(cpy(st, st->d_row_off, roff.data(), nrows * sizeof(int), hipMemcpyHostToDevice));
+    PS_TRY_HIP(cpy(st, st->d_row_len, rlen.data(), nrows * sizeof(int), hipMemcpyHostToDevice));
+    PS_TRY_HIP(cpy(st, st->d_pair_bc, pbc.data(), pbc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    PS_TRY_HIP(cpy(st, st->d_tasks, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
+    PS_TRY_HIP(cpy(st, st->d_wave_tasks, wt.data(), wt.size() * sizeof(int), hipMemcpyHostToDevice));
     return PSPIN_OK;
 }
 
@@ -400,6 +484,10 @@ int pspin_create(const pspin_sim_config* cfg, int variant, pspin_state** out) {
                         terms.push_back(Term{{i, j, k, l}, ch, hg[q]});
                     }
         st->n_sq = (int)std::min<long long>(nq, 2147483647LL);
+        if (cfg->fmc_mode == 0 && N >= 8) {
+            if ((rc = build_pair_layout(st, hg, hm))) { (void)hipFree(d_omega); return fail(rc); }
+            st->kernel = PSPIN_KERNEL_PAIRS;
+        }
     } else {
         // Sparse: unit g4 + mask, survivors in ascending q*3+ch, partial
         // Fisher-Yates with splitmix64(seed+4000), N quartets
@@ -643,7 +731,8 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
     PS_REQUIRE(st && beta_sched, PSPIN_ERR_ARG, "null argument");
     PS_REQUIRE(NT >= 2 && nrep_per_temp >= 1 && (long long)NT * nrep_per_temp == st->nrep, PSPIN_ERR_ARG,
                "NT * nrep_per_temp must equal nrep and NT >= 2");
-    PS_REQUIRE((size_t)NT * nrep_per_temp * sizeof(int) <= 60 * 1024, PSPIN_ERR_ARG, "too many replicas for the exchange kernel");
+    PS_REQUIRE((size_t)NT * nrep_per_temp * 12 + (size_t)NT * 8 <= 60 * 1024, PSPIN_ERR_ARG,
+               "too many replicas for the exchange kernel (NT * nrep_per_temp <= 5000)");
     dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48);
     st->pt_NT = NT;
     st->pt_nrep = nrep_per_temp;
@@ -817,10 +906,13 @@ int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweep, dou
 
 int pspin_set_kernel(pspin_state* st, int kernel, int block_size) {
     PS_REQUIRE(st, PSPIN_ERR_ARG, "null argument");
-    PS_REQUIRE(kernel == PSPIN_KERNEL_AUTO || kernel == PSPIN_KERNEL_LISTS, PSPIN_ERR_ARG, "kernel not available");
+    PS_REQUIRE(kernel == PSPIN_KERNEL_AUTO || kernel == PSPIN_KERNEL_LISTS || kernel == PSPIN_KERNEL_PAIRS,
+               PSPIN_ERR_ARG, "unknown kernel");
+    PS_REQUIRE(kernel != PSPIN_KERNEL_PAIRS || st->d_ptab, PSPIN_ERR_ARG,
+               "PSPIN_KERNEL_PAIRS needs a dense fully connected system (fmc_mode 0, N >= 8)");
     PS_REQUIRE(block_size == 0 || (block_size % 64 == 0 && block_size >= 64 && block_size <= 1024), PSPIN_ERR_ARG,
                "block_size must be 0 or a multiple of 64 in [64, 1024]");
-    st->kernel = kernel == PSPIN_KERNEL_AUTO ? PSPIN_KERNEL_LISTS : kernel;
+    st->kernel = kernel == PSPIN_KERNEL_AUTO ? (st->d_ptab ? PSPIN_KERNEL_PAIRS : PSPIN_KERNEL_LISTS) : kernel;
     st->block_size = block_size;
     return PSPIN_OK;
 }

@@ -26,21 +26,29 @@ struct ExArgs {
 };
 
 __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
-    extern __shared__ int sperm[];
+    // LDS: energies and betas (read once, coalesced), then the permutation.
+    extern __shared__ double lds[];
     const int lane = threadIdx.x;
     const int NT = a.NT, nr = a.nrep, n = NT * nr;
-    for (int i = lane; i < n; i += 64) sperm[i] = a.perm[i];
+    double* sE = lds;
+    double* sB = lds + n;
+    int* sperm = (int*)(lds + n + NT);
+    for (int i = lane; i < n; i += 64) {
+        sE[i] = a.E[i];
+        sperm[i] = a.perm[i];
+    }
+    for (int i = lane; i < NT; i += 64) sB[i] = a.beta_sched[i];
     __syncthreads();
     uint64_t X = *a.rng48;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int t = 0; t < NT - 1; t++) {
-        const double db = a.beta_sched[t] - a.beta_sched[t + 1];
+        const double db = sB[t] - sB[t + 1];
         long long nacc = 0;
         for (int c0 = 0; c0 < nr; c0 += 64) {
             const int r = c0 + lane;
             const bool valid = r < nr;
             double delta = 0.0;
-            if (valid) delta = db * (a.E[sperm[t * nr + r]] - a.E[sperm[(t + 1) * nr + r]]);
+            if (valid) delta = db * (sE[sperm[t * nr + r]] - sE[sperm[(t + 1) * nr + r]]);
             const bool need = valid && !(delta >= 0.0);
             const unsigned long long m = __ballot(need);
             bool accept = valid;
@@ -67,12 +75,12 @@ __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
     __syncthreads();
     for (int i = lane; i < n; i += 64) {
         a.perm[i] = sperm[i];
-        a.betas[sperm[i]] = a.beta_sched[i / nr];
+        a.betas[sperm[i]] = sB[i / nr];
     }
     if (a.series) {
         const long long round = *a.series_n;
         double* row = a.series + (size_t)(round % a.series_cap) * nr;
-        for (int r = lane; r < nr; r += 64) row[r] = a.E[sperm[(NT - 1) * nr + r]];
+        for (int r = lane; r < nr; r += 64) row[r] = sE[sperm[(NT - 1) * nr + r]];
     }
     __syncthreads();
     if (lane == 0) {
@@ -94,7 +102,7 @@ hipError_t launch_exchange(pspin_state* st) {
     a.series = st->series_cap > 0 ? st->d_series : nullptr;
     a.series_n = st->d_series_n;
     a.series_cap = st->series_cap;
-    size_t lds = (size_t)a.NT * a.nrep * sizeof(int);
+    size_t lds = (size_t)a.NT * a.nrep * (sizeof(double) + sizeof(int)) + (size_t)a.NT * sizeof(double);
     hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), lds, st->stream, a);
     return hipGetLastError();
 }

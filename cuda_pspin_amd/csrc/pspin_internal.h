@@ -160,6 +160,17 @@ struct pspin_state {
     long long n_ent = 0;
     int max_site_len = 0;
 
+    // dense fully connected per-site pair-row slices (PSPIN_KERNEL_PAIRS)
+    float* d_ptab = nullptr;      // [N][pair_slice_len]
+    int pair_slice_len = 0;
+    int* d_row_off = nullptr;     // [N-3]
+    int* d_row_len = nullptr;     // [N-3]
+    uint32_t* d_pair_bc = nullptr;
+    int pair_npairs = 0;
+    int4* d_tasks = nullptr;
+    int* d_wave_tasks = nullptr;
+    int pair_nwaves = 0;
+
     // replicas
     double* d_spins = nullptr;  // [nrep][N][2]
     double* d_E = nullptr;

@@ -19,8 +19,20 @@
 // that contain both sites, and in differential form, so FP32 arithmetic does
 // not cancel new-minus-old products.
 //
-// Couplings of site x are a contiguous list of (p | m<<8 | n<<16, g) entries:
-// coalesced 8-byte loads; the three amplitudes come from LDS.
+// Two coupling layouts feed the same step machinery:
+//  LISTS  site x owns a contiguous list of (p | m<<8 | n<<16, g) entries
+//         (any channel pattern: FMC, comb, sparse quartets); coalesced 8-byte
+//         loads, the three amplitudes from LDS.
+//  PAIRS  fully connected (every quartet channel 0): site x owns a dense
+//         FP32 slice indexed by (row a, pair (b,c)) over the other sites
+//         relabelled 0..N-2, rows a < b < c.  Pairs are ordered by b
+//         descending so row a is a prefix; a lane owns 4 consecutive pairs
+//         and walks rows with one 16-byte load per row, accumulating
+//         W_bc = sum_a g_abc t_a against the broadcast amplitude t_a.  The
+//         local field is then sum_bc U_bc W_bc (partner a) or U_bc conj(W_bc)
+//         (partner c) with U_bc a product of two amplitudes: 4 bytes and 2 FMA
+//         per coupling, no index data.  A static task list (chunk block x
+//         row range, longest-first to waves) balances the triangular rows.
 //
 // Randomness: Philox4x32-10 keyed by the replica seed (the reference's
 // splitmix64 chain, src/mc.cu:418-423) and countered by (sweep, step), all N/2
@@ -46,6 +58,15 @@ struct SweepArgs {
     StepRec* log;
     int log_cap;   // sweeps per replica in the log
     int log_base;  // sweeps already logged
+    // PAIRS layout
+    const float* ptab;        // [N][slice_len]
+    int slice_len;
+    const int* row_off;       // [N-3] offset of row a in a slice (floats)
+    const int* row_len;       // [N-3] length of row a, multiple of 4
+    const uint32_t* pair_bc;  // [npairs] b | c << 16
+    int npairs;
+    const int4* tasks;        // {slot (0: i0, 1: j0), first chunk, a0, a1}
+    const int* wave_tasks;    // [nwaves+1] task ranges per wave
 };
 
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
@@ -88,8 +109,106 @@ __device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t sweep, int 
     return p;
 }
 
-template <bool CUBE>
-__global__ __launch_bounds__(1024) void k_sweep_lists(SweepArgs a) {
+// Local-field contribution of the thread for the current step, LISTS layout.
+__device__ __forceinline__ float field_lists(const SweepArgs& a, const float2* s32, int i0, int j0, float2 di,
+                                             float2 dj, int tid, int bdim) {
+    float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
+    const int oi = a.site_off[i0], ni_ = a.site_off[i0 + 1] - oi;
+    const int oj = a.site_off[j0], nj_ = a.site_off[j0 + 1] - oj;
+#pragma unroll 4
+    for (int e = tid; e < ni_; e += bdim) {
+        const uint2 en = a.site_ent[oi + e];
+        const float2 tp = s32[en.x & 0xff], tm = s32[(en.x >> 8) & 0xff], tn = s32[(en.x >> 16) & 0xff];
+        const float g = __uint_as_float(en.y);
+        const float2 mn = cmulf(tm, tn);
+        // X = tp * conj(tm tn)
+        Fi.x += g * (tp.x * mn.x + tp.y * mn.y);
+        Fi.y += g * (tp.y * mn.x - tp.x * mn.y);
+    }
+#pragma unroll 4
+    for (int e = tid; e < nj_; e += bdim) {
+        const uint2 en = a.site_ent[oj + e];
+        const float2 tp = s32[en.x & 0xff], tm = s32[(en.x >> 8) & 0xff], tn = s32[(en.x >> 16) & 0xff];
+        const float g = __uint_as_float(en.y);
+        const float2 mn = cmulf(tm, tn);
+        Fj.x += g * (tp.x * mn.x + tp.y * mn.y);
+        Fj.y += g * (tp.y * mn.x - tp.x * mn.y);
+    }
+    return -(di.x * Fi.x - di.y * Fi.y) - (dj.x * Fj.x - dj.y * Fj.y);
+}
+
+// Local-field contribution of the thread for the current step, PAIRS layout
+// (fully connected, channel 0: f = a_p0 a_p1 conj(a_p2) conj(a_p3)).
+// For site x and others a<b<c: partner is a when o(b) > x, else c.
+__device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s32, int i0, int j0, float2 di,
+                                             float2 dj, int w, int lane) {
+    float local = 0.f;
+    const int t_end = a.wave_tasks[w + 1];
+    for (int t = a.wave_tasks[w]; t < t_end; t++) {
+        const int4 tk = a.tasks[t];
+        const int site = tk.x ? j0 : i0;
+        const float2 d = tk.x ? dj : di;
+        const int P0 = 4 * (tk.y + lane);
+        float2 U[4];
+        bool pa[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            U[j] = make_float2(0.f, 0.f);
+            pa[j] = true;
+            if (P0 + j < a.npairs) {
+                const uint32_t bc = a.pair_bc[P0 + j];
+                const int b = bc & 0xffff, c = bc >> 16;
+                const float2 tb = s32[b + (b >= site)], tc = s32[c + (c >= site)];
+                pa[j] = b >= site;
+                // partner a: U = conj(tb tc); partner c: U = conj(tb) tc
+                U[j] = pa[j] ? make_float2(tb.x * tc.x - tb.y * tc.y, -(tb.x * tc.y + tb.y * tc.x))
+                             : make_float2(tb.x * tc.x + tb.y * tc.y, tb.x * tc.y - tb.y * tc.x);
+            }
+        }
+        float2 acc[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[j] = make_float2(0.f, 0.f);
+        const float* base = a.ptab + (size_t)site * a.slice_len + P0;
+        int r = tk.z;
+        for (; r + 4 <= tk.w; r += 4) {
+            float4 g[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                g[u] = (P0 < a.row_len[r + u]) ? *(const float4*)(base + a.row_off[r + u]) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int ar = r + u;
+                const float2 ta = s32[ar + (ar >= site)];
+                acc[0].x += g[u].x * ta.x; acc[0].y += g[u].x * ta.y;
+                acc[1].x += g[u].y * ta.x; acc[1].y += g[u].y * ta.y;
+                acc[2].x += g[u].z * ta.x; acc[2].y += g[u].z * ta.y;
+                acc[3].x += g[u].w * ta.x; acc[3].y += g[u].w * ta.y;
+            }
+        }
+        for (; r < tk.w; r++) {
+            if (P0 < a.row_len[r]) {
+                const float4 g = *(const float4*)(base + a.row_off[r]);
+                const float2 ta = s32[r + (r >= site)];
+                acc[0].x += g.x * ta.x; acc[0].y += g.x * ta.y;
+                acc[1].x += g.y * ta.x; acc[1].y += g.y * ta.y;
+                acc[2].x += g.z * ta.x; acc[2].y += g.z * ta.y;
+                acc[3].x += g.w * ta.x; acc[3].y += g.w * ta.y;
+            }
+        }
+        float2 H = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float2 W = pa[j] ? acc[j] : make_float2(acc[j].x, -acc[j].y);
+            H.x += U[j].x * W.x - U[j].y * W.y;
+            H.y += U[j].x * W.y + U[j].y * W.x;
+        }
+        local -= d.x * H.x - d.y * H.y;
+    }
+    return local;
+}
+
+template <bool CUBE, bool PAIRS>
+__global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     __shared__ double2 s64[kMaxN];
     __shared__ float2 s32[kMaxN];
     __shared__ StepPre pre[kMaxN / 2];
@@ -181,8 +300,9 @@ __global__ __launch_bounds__(1024) void k_sweep_lists(SweepArgs a) {
 
             const int i0 = pair[0], j0 = pair[1];
             const float2 di = dlt[0], dj = dlt[1];
-            float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
+            float local = 0.f;
             if (a.h2_active) {
+                float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
                 const float* ri = a.g2 + (size_t)i0 * N;
                 const float* rj = a.g2 + (size_t)j0 * N;
                 for (int k = tid; k < N; k += bdim) {
@@ -190,27 +310,12 @@ __global__ __launch_bounds__(1024) void k_sweep_lists(SweepArgs a) {
                     if (k != i0) { const float g = ri[k]; Fi.x += g * t.x; Fi.y -= g * t.y; }
                     if (k != j0) { const float g = rj[k]; Fj.x += g * t.x; Fj.y -= g * t.y; }
                 }
+                local = -(di.x * Fi.x - di.y * Fi.y) - (dj.x * Fj.x - dj.y * Fj.y);
             }
-            const int oi = a.site_off[i0], ni_ = a.site_off[i0 + 1] - oi;
-            const int oj = a.site_off[j0], nj_ = a.site_off[j0 + 1] - oj;
-            for (int e = tid; e < ni_; e += bdim) {
-                const uint2 en = a.site_ent[oi + e];
-                const float2 tp = s32[en.x & 0xff], tm = s32[(en.x >> 8) & 0xff], tn = s32[(en.x >> 16) & 0xff];
-                const float g = __uint_as_float(en.y);
-                const float2 mn = cmulf(tm, tn);
-                // X = tp * conj(tm tn)
-                Fi.x += g * (tp.x * mn.x + tp.y * mn.y);
-                Fi.y += g * (tp.y * mn.x - tp.x * mn.y);
-            }
-            for (int e = tid; e < nj_; e += bdim) {
-                const uint2 en = a.site_ent[oj + e];
-                const float2 tp = s32[en.x & 0xff], tm = s32[(en.x >> 8) & 0xff], tn = s32[(en.x >> 16) & 0xff];
-                const float g = __uint_as_float(en.y);
-                const float2 mn = cmulf(tm, tn);
-                Fj.x += g * (tp.x * mn.x + tp.y * mn.y);
-                Fj.y += g * (tp.y * mn.x - tp.x * mn.y);
-            }
-            float local = -(di.x * Fi.x - di.y * Fi.y) - (dj.x * Fj.x - dj.y * Fj.y);
+            if (PAIRS)
+                local += field_pairs(a, s32, i0, j0, di, dj, w, lane);
+            else
+                local += field_lists(a, s32, i0, j0, di, dj, tid, bdim);
             local = wave_sum(local);
             if (lane == 0) part[step & 1][w] = local;
             __syncthreads();
@@ -247,16 +352,27 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
     a.log = st->d_log;
     a.log_cap = st->log_cap;
     a.log_base = st->log_n;
-    int bs = st->block_size;
+    a.ptab = st->d_ptab;
+    a.slice_len = st->pair_slice_len;
+    a.row_off = st->d_row_off;
+    a.row_len = st->d_row_len;
+    a.pair_bc = st->d_pair_bc;
+    a.npairs = st->pair_npairs;
+    a.tasks = st->d_tasks;
+    a.wave_tasks = st->d_wave_tasks;
+    const bool pairs = st->kernel == PSPIN_KERNEL_PAIRS;
+    int bs = pairs ? 64 * st->pair_nwaves : st->block_size;
     if (bs <= 0) {
         // Enough threads that each touches a few entries per step.
         long long per_step = 2LL * st->max_site_len;
         bs = per_step >= 8192 ? 1024 : per_step >= 1024 ? 256 : 64;
     }
     if (st->variant == PSPIN_SPARSE)
-        hipLaunchKernelGGL(k_sweep_lists<true>, dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        hipLaunchKernelGGL((k_sweep<true, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+    else if (pairs)
+        hipLaunchKernelGGL((k_sweep<false, true>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
     else
-        hipLaunchKernelGGL(k_sweep_lists<false>, dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        hipLaunchKernelGGL((k_sweep<false, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
         st->sweep_counter += (uint64_t)n_sweeps;

@@ -49,8 +49,8 @@ def test_exchange_bit_exact(pspin, oracle, NT, nrep):
 
 def test_reference_replica_exchange(pspin, oracle):
     # src/test_replica_exchange.cu with run_tests.sh "8 6 65536 5" (seed 65432,
-    # linear T in [0.5, 2], srand48(seed+7777)); 16384 sweeps here.
-    N, NT, sweeps, pt_freq = 8, 6, 16384, 5
+    # linear T in [0.5, 2], srand48(seed+7777)); 16380 sweeps here.
+    N, NT, sweeps, pt_freq = 8, 6, 16380, 5
     cfg = pspin.SimConfig.default(N=N, nrep=NT, T=2.0, J=2.0, J0=0.0, alpha=0.5, alpha0=0.0, seed=65432)
     st = pspin.PSpin(cfg, 0)
     T, beta = oracle.temperatures(NT, 0.5, 2.0, 0)

@@ -101,18 +101,25 @@ def replay_check(O, st, n_sweeps, betas):
     return dict(worst_scaled=worst, worst_rel=worst_rel, acc_rate=n_acc / (nrep * n_sweeps * steps))
 
 
-@pytest.mark.parametrize("name,variant,kw,T,nsw", [
-    ("dense FC N=12", 0, dict(N=12, fmc_mode=0), 1.0, 6),
-    ("dense uniform N=32 (config-1 system)", 0, dict(N=32, fmc_mode=2, gamma=0.05), 0.4, 3),
-    ("dense comb N=16", 0, dict(N=16, fmc_mode=1, gamma=1.0), 0.8, 4),
-    ("dense FC N=20 J0", 0, dict(N=20, fmc_mode=0, J0=0.4, alpha0=0.5), 0.6, 2),
-    ("sparse cube N=16", 1, dict(N=16, fmc_mode=2, gamma=0.05), 0.5, 10),
-    ("sparse cube FC N=24", 1, dict(N=24, fmc_mode=0), 1.0, 6),
+@pytest.mark.parametrize("name,variant,kernel,kw,T,nsw", [
+    ("dense FC N=12 pairs", 0, 2, dict(N=12, fmc_mode=0), 1.0, 6),
+    ("dense FC N=12 lists", 0, 1, dict(N=12, fmc_mode=0), 1.0, 6),
+    ("dense uniform N=32 (config-1 system)", 0, 1, dict(N=32, fmc_mode=2, gamma=0.05), 0.4, 3),
+    ("dense comb N=16", 0, 1, dict(N=16, fmc_mode=1, gamma=1.0), 0.8, 4),
+    ("dense FC N=20 J0 pairs", 0, 2, dict(N=20, fmc_mode=0, J0=0.4, alpha0=0.5), 0.6, 2),
+    ("dense FC N=64 pairs (bench system)", 0, 2, dict(N=64, fmc_mode=0), 0.3, 2),
+    ("dense FC N=64 lists", 0, 1, dict(N=64, fmc_mode=0), 0.3, 1),
+    ("dense uniform N=64 (learned-proposal system)", 0, 1, dict(N=64, fmc_mode=2, gamma=0.05), 0.3, 1),
+    ("dense FC N=33 odd pairs", 0, 2, dict(N=33, fmc_mode=0), 0.7, 2),
+    ("sparse cube N=16", 1, 1, dict(N=16, fmc_mode=2, gamma=0.05), 0.5, 10),
+    ("sparse cube FC N=24", 1, 1, dict(N=24, fmc_mode=0), 1.0, 6),
 ])
-def test_step_replay(pspin, oracle, name, variant, kw, T, nsw):
+def test_step_replay(pspin, oracle, name, variant, kernel, kw, T, nsw):
     print(f"[step replay] {name}", flush=True)
     cfg = pspin.SimConfig.default(nrep=4, seed=1234, **kw)
     st = pspin.PSpin(cfg, variant)
+    st.set_kernel(kernel, 0)
+    assert st.info.kernel == kernel
     betas = np.array([1.0 / T, 2.0 / T, 0.5 / T, 0.0])
     stats = replay_check(oracle, st, nsw, betas)
     print(f"  max |dE32-dE64|/S = {stats['worst_scaled']:.2e}  vs |dE64|: {stats['worst_rel']:.2e}  "
@@ -185,28 +192,58 @@ def test_reference_inf_temp(pspin):
     st.close()
 
 
+def _mean_se_batches(x, nb=20):
+    """Mean and batch-means standard error of a per-sample series [n_samples, n_chains]."""
+    x = np.asarray(x)
+    b = np.array_split(x, nb, axis=0)
+    m = np.array([bb.mean() for bb in b])
+    return float(x.mean()), float(m.std(ddof=1) / math.sqrt(nb))
+
+
+def _run_group(pspin, seed, N, nrep, T, equil, measure, every=5):
+    cfg = pspin.SimConfig.default(N=N, nrep=nrep, T=T, J=2.0, J0=0.0, alpha=0.5, alpha0=0.0, seed=seed)
+    st = pspin.PSpin(cfg, 0)
+    st.set_uniform_beta(1.0 / T)
+    st.sweep(equil)
+    out = []
+    for _ in range(measure // every):
+        st.sweep(every)
+        out.append(st.results()[0] / N)
+    st.close()
+    return np.array(out)  # [samples, replicas]
+
+
 def test_reference_detailed_balance(pspin):
-    # src/test_detailed_balance.cu "8 8 500 1000": two disorder samples at
-    # T=0.8 agree within 5 combined standard errors; both variances > 0.
-    N, nrep, equil, measure, T = 8, 8, 500, 1000, 0.8
-    sts = []
-    for seed in (11111, 99999):
-        cfg = pspin.SimConfig.default(N=N, nrep=nrep, T=T, J=2.0, J0=0.0, alpha=0.5, alpha0=0.0, seed=seed)
-        st = pspin.PSpin(cfg, 0)
-        st.set_uniform_beta(1.0 / T)
-        st.sweep(equil)
-        sts.append(st)
-    samples = [[], []]
-    for _ in range(measure // 5):
-        for g, st in enumerate(sts):
-            st.sweep(5)
-            samples[g].extend((st.results()[0] / N).tolist())
-    a, b = np.array(samples[0]), np.array(samples[1])
-    se = math.sqrt(a.var() / len(a) + b.var() / len(b))
-    assert abs(a.mean() - b.mean()) / se < 5.0
+    """src/test_detailed_balance.cu "8 8 500 1000" (N=8, 8 replicas per group, T=0.8).
+
+    Same disorder (the check a Markov chain with the right stationary law must
+    pass): replicas 0-7 and 8-15 of one system are independent chains and
+    their mean energies agree within 5 batch-means standard errors.
+
+    As written in the reference the two groups use different disorder
+    (seeds 11111 and 99999, src/test_detailed_balance.cu:58-71), so their
+    means differ by the sample-to-sample spread of <e> at N=8, which the
+    reference's 5-sigma check (thermal error only) does not include.  Here the
+    as-written difference is checked against the thermal error plus the
+    disorder spread measured over 12 further samples; both variances > 0.
+    """
+    N, T = 8, 0.8
+    x = _run_group(pspin, 11111, N, 16, T, 500, 1000)
+    a, b = x[:, :8], x[:, 8:]
+    ma, sa = _mean_se_batches(a)
+    mb, sb = _mean_se_batches(b)
+    z = abs(ma - mb) / math.sqrt(sa ** 2 + sb ** 2)
+    print(f"  same disorder: {ma:.5f} +- {sa:.5f} vs {mb:.5f} +- {sb:.5f}  ({z:.2f} sigma)")
+    assert z < 5.0
     assert a.var() > 0 and b.var() > 0
-    for st in sts:
-        st.close()
+    xb = _run_group(pspin, 99999, N, 8, T, 500, 1000)
+    mB, sB = _mean_se_batches(xb)
+    means = [_run_group(pspin, 20000 + k, N, 8, T, 300, 300).mean() for k in range(12)]
+    sd_dis = float(np.std(means, ddof=1))
+    z2 = abs(ma - mB) / math.sqrt(sa ** 2 + sB ** 2 + 2 * sd_dis ** 2)
+    print(f"  as written: {ma:.5f} vs {mB:.5f}; disorder spread {sd_dis:.4f} ({z2:.2f} sigma)")
+    assert z2 < 5.0
+    assert xb.var() > 0
 
 
 def test_reference_sparse_dense(pspin, oracle):
