@@ -172,7 +172,7 @@ int build_pair_layout(pspin_state* st, const std::vector<double>& g4, const std:
     }
     // tasks: blocks of 64 chunks of 4 pairs; rows [0, b_max(block))
     const int nchunks = (npairs + 3) / 4, nblocks = (nchunks + 63) / 64;
-    constexpr int kRows = 6;
+    constexpr int kRows = 8;
     std::vector<std::array<int, 4>> tasks;
     for (int slot = 0; slot < 2; slot++)
         for (int blk = 0; blk < nblocks; blk++) {
@@ -484,7 +484,7 @@ int pspin_create(const pspin_sim_config* cfg, int variant, pspin_state** out) {
                         terms.push_back(Term{{i, j, k, l}, ch, hg[q]});
                     }
         st->n_sq = (int)std::min<long long>(nq, 2147483647LL);
-        if (cfg->fmc_mode == 0 && N >= 8) {
+        if (cfg->fmc_mode == 0 && N >= 8 && N <= ps::kMaxPairsN) {
             if ((rc = build_pair_layout(st, hg, hm))) { (void)hipFree(d_omega); return fail(rc); }
             st->kernel = PSPIN_KERNEL_PAIRS;
         }
@@ -731,7 +731,7 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
     PS_REQUIRE(st && beta_sched, PSPIN_ERR_ARG, "null argument");
     PS_REQUIRE(NT >= 2 && nrep_per_temp >= 1 && (long long)NT * nrep_per_temp == st->nrep, PSPIN_ERR_ARG,
                "NT * nrep_per_temp must equal nrep and NT >= 2");
-    PS_REQUIRE((size_t)NT * nrep_per_temp * 12 + (size_t)NT * 8 <= 60 * 1024, PSPIN_ERR_ARG,
+    PS_REQUIRE((size_t)NT * nrep_per_temp * 12 + (size_t)NT * 16 <= 60 * 1024, PSPIN_ERR_ARG,
                "too many replicas for the exchange kernel (NT * nrep_per_temp <= 5000)");
     dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48);
     st->pt_NT = NT;
@@ -904,12 +904,28 @@ int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweep, dou
     return PSPIN_OK;
 }
 
+int pspin_device_info(int dev, size_t* free_bytes, size_t* total_bytes, char* name, int name_len) {
+    int cur = 0;
+    PS_TRY_HIP(hipGetDevice(&cur));
+    if (dev >= 0 && dev != cur) PS_TRY_HIP(hipSetDevice(dev));
+    size_t f = 0, t = 0;
+    PS_TRY_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    if (name && name_len > 0) {
+        hipDeviceProp_t prop;
+        PS_TRY_HIP(hipGetDeviceProperties(&prop, dev >= 0 ? dev : cur));
+        std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    return PSPIN_OK;
+}
+
 int pspin_set_kernel(pspin_state* st, int kernel, int block_size) {
     PS_REQUIRE(st, PSPIN_ERR_ARG, "null argument");
     PS_REQUIRE(kernel == PSPIN_KERNEL_AUTO || kernel == PSPIN_KERNEL_LISTS || kernel == PSPIN_KERNEL_PAIRS,
                PSPIN_ERR_ARG, "unknown kernel");
     PS_REQUIRE(kernel != PSPIN_KERNEL_PAIRS || st->d_ptab, PSPIN_ERR_ARG,
-               "PSPIN_KERNEL_PAIRS needs a dense fully connected system (fmc_mode 0, N >= 8)");
+               "PSPIN_KERNEL_PAIRS needs a dense fully connected system (fmc_mode 0, 8 <= N <= 128)");
     PS_REQUIRE(block_size == 0 || (block_size % 64 == 0 && block_size >= 64 && block_size <= 1024), PSPIN_ERR_ARG,
                "block_size must be 0 or a multiple of 64 in [64, 1024]");
     st->kernel = kernel == PSPIN_KERNEL_AUTO ? (st->d_ptab ? PSPIN_KERNEL_PAIRS : PSPIN_KERNEL_LISTS) : kernel;

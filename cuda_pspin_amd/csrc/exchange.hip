@@ -32,7 +32,8 @@ __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
     const int NT = a.NT, nr = a.nrep, n = NT * nr;
     double* sE = lds;
     double* sB = lds + n;
-    int* sperm = (int*)(lds + n + NT);
+    long long* sacc = (long long*)(lds + n + NT);  // [NT-1] accepted swaps this round
+    int* sperm = (int*)(sacc + NT);
     for (int i = lane; i < n; i += 64) {
         sE[i] = a.E[i];
         sperm[i] = a.perm[i];
@@ -65,14 +66,16 @@ __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
             nacc += __popcll(__ballot(valid && accept));
             __syncthreads();
         }
-        if (lane == 0) {
-            a.ex[0 * (NT - 1) + t] += nacc;
-            a.ex[1 * (NT - 1) + t] += nr;
-            a.ex[2 * (NT - 1) + t] += nacc;
-            a.ex[3 * (NT - 1) + t] += nr;
-        }
+        if (lane == 0) sacc[t] = nacc;
     }
     __syncthreads();
+    // One independent read-modify-write per counter, all lanes at once.
+    for (int t = lane; t < NT - 1; t += 64) {
+        a.ex[0 * (NT - 1) + t] += sacc[t];
+        a.ex[1 * (NT - 1) + t] += nr;
+        a.ex[2 * (NT - 1) + t] += sacc[t];
+        a.ex[3 * (NT - 1) + t] += nr;
+    }
     for (int i = lane; i < n; i += 64) {
         a.perm[i] = sperm[i];
         a.betas[sperm[i]] = sB[i / nr];
@@ -102,7 +105,7 @@ hipError_t launch_exchange(pspin_state* st) {
     a.series = st->series_cap > 0 ? st->d_series : nullptr;
     a.series_n = st->d_series_n;
     a.series_cap = st->series_cap;
-    size_t lds = (size_t)a.NT * a.nrep * (sizeof(double) + sizeof(int)) + (size_t)a.NT * sizeof(double);
+    size_t lds = (size_t)a.NT * a.nrep * (sizeof(double) + sizeof(int)) + (size_t)a.NT * (sizeof(double) + 8);
     hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), lds, st->stream, a);
     return hipGetLastError();
 }

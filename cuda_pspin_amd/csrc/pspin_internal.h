@@ -17,6 +17,7 @@
 namespace ps {
 
 constexpr int kMaxN = 256;         // site indices are packed in 8 bits
+constexpr int kMaxPairsN = 128;    // largest N served by the dense pair-row kernel
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr uint32_t kTagSpins = 1;  // Philox stream tags (second key word is the seed)
 constexpr uint32_t kTagG2 = 2;

@@ -110,29 +110,36 @@ __device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t sweep, int 
 }
 
 // Local-field contribution of the thread for the current step, LISTS layout.
+// The two site lists are walked as one range; each thread issues kBatch
+// independent 8-byte loads before using any, so enough bytes are in flight
+// to cover L2/MALL latency.  Entries past the end read as g = 0.
+constexpr int kListBatch = 8;
 __device__ __forceinline__ float field_lists(const SweepArgs& a, const float2* s32, int i0, int j0, float2 di,
                                              float2 dj, int tid, int bdim) {
     float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
     const int oi = a.site_off[i0], ni_ = a.site_off[i0 + 1] - oi;
     const int oj = a.site_off[j0], nj_ = a.site_off[j0 + 1] - oj;
-#pragma unroll 4
-    for (int e = tid; e < ni_; e += bdim) {
-        const uint2 en = a.site_ent[oi + e];
-        const float2 tp = s32[en.x & 0xff], tm = s32[(en.x >> 8) & 0xff], tn = s32[(en.x >> 16) & 0xff];
-        const float g = __uint_as_float(en.y);
-        const float2 mn = cmulf(tm, tn);
-        // X = tp * conj(tm tn)
-        Fi.x += g * (tp.x * mn.x + tp.y * mn.y);
-        Fi.y += g * (tp.y * mn.x - tp.x * mn.y);
-    }
-#pragma unroll 4
-    for (int e = tid; e < nj_; e += bdim) {
-        const uint2 en = a.site_ent[oj + e];
-        const float2 tp = s32[en.x & 0xff], tm = s32[(en.x >> 8) & 0xff], tn = s32[(en.x >> 16) & 0xff];
-        const float g = __uint_as_float(en.y);
-        const float2 mn = cmulf(tm, tn);
-        Fj.x += g * (tp.x * mn.x + tp.y * mn.y);
-        Fj.y += g * (tp.y * mn.x - tp.x * mn.y);
+    const int n_tot = ni_ + nj_;
+    for (int e0 = tid; e0 < n_tot; e0 += kListBatch * bdim) {
+        uint2 en[kListBatch];
+#pragma unroll
+        for (int u = 0; u < kListBatch; u++) {
+            const int e = e0 + u * bdim;
+            en[u] = (e < n_tot) ? a.site_ent[e < ni_ ? oi + e : oj + (e - ni_)] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < kListBatch; u++) {
+            const float2 tp = s32[en[u].x & 0xff], tm = s32[(en[u].x >> 8) & 0xff], tn = s32[(en[u].x >> 16) & 0xff];
+            const float g = __uint_as_float(en[u].y);
+            const float2 mn = cmulf(tm, tn);
+            // g X with X = tp * conj(tm tn)
+            const float xr = g * (tp.x * mn.x + tp.y * mn.y), xi = g * (tp.y * mn.x - tp.x * mn.y);
+            const bool isj = e0 + u * bdim >= ni_;
+            Fi.x += isj ? 0.f : xr;
+            Fi.y += isj ? 0.f : xi;
+            Fj.x += isj ? xr : 0.f;
+            Fj.y += isj ? xi : 0.f;
+        }
     }
     return -(di.x * Fi.x - di.y * Fi.y) - (dj.x * Fj.x - dj.y * Fj.y);
 }
@@ -140,8 +147,10 @@ __device__ __forceinline__ float field_lists(const SweepArgs& a, const float2* s
 // Local-field contribution of the thread for the current step, PAIRS layout
 // (fully connected, channel 0: f = a_p0 a_p1 conj(a_p2) conj(a_p3)).
 // For site x and others a<b<c: partner is a when o(b) > x, else c.
-__device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s32, int i0, int j0, float2 di,
-                                             float2 dj, int w, int lane) {
+// Rows are consumed kPairBatch at a time with all loads issued first.
+constexpr int kPairBatch = 8;
+__device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s32, const uint32_t* pbc, int i0,
+                                             int j0, float2 di, float2 dj, int w, int lane) {
     float local = 0.f;
     const int t_end = a.wave_tasks[w + 1];
     for (int t = a.wave_tasks[w]; t < t_end; t++) {
@@ -156,7 +165,7 @@ __device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s
             U[j] = make_float2(0.f, 0.f);
             pa[j] = true;
             if (P0 + j < a.npairs) {
-                const uint32_t bc = a.pair_bc[P0 + j];
+                const uint32_t bc = pbc[P0 + j];
                 const int b = bc & 0xffff, c = bc >> 16;
                 const float2 tb = s32[b + (b >= site)], tc = s32[c + (c >= site)];
                 pa[j] = b >= site;
@@ -169,30 +178,22 @@ __device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[j] = make_float2(0.f, 0.f);
         const float* base = a.ptab + (size_t)site * a.slice_len + P0;
-        int r = tk.z;
-        for (; r + 4 <= tk.w; r += 4) {
-            float4 g[4];
+        for (int r = tk.z; r < tk.w; r += kPairBatch) {
+            float4 g[kPairBatch];
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                g[u] = (P0 < a.row_len[r + u]) ? *(const float4*)(base + a.row_off[r + u]) : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < kPairBatch; u++) {
                 const int ar = r + u;
+                g[u] = (ar < tk.w && P0 < a.row_len[ar]) ? *(const float4*)(base + a.row_off[ar])
+                                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < kPairBatch; u++) {
+                const int ar = min(r + u, tk.w - 1);
                 const float2 ta = s32[ar + (ar >= site)];
                 acc[0].x += g[u].x * ta.x; acc[0].y += g[u].x * ta.y;
                 acc[1].x += g[u].y * ta.x; acc[1].y += g[u].y * ta.y;
                 acc[2].x += g[u].z * ta.x; acc[2].y += g[u].z * ta.y;
                 acc[3].x += g[u].w * ta.x; acc[3].y += g[u].w * ta.y;
-            }
-        }
-        for (; r < tk.w; r++) {
-            if (P0 < a.row_len[r]) {
-                const float4 g = *(const float4*)(base + a.row_off[r]);
-                const float2 ta = s32[r + (r >= site)];
-                acc[0].x += g.x * ta.x; acc[0].y += g.x * ta.y;
-                acc[1].x += g.y * ta.x; acc[1].y += g.y * ta.y;
-                acc[2].x += g.z * ta.x; acc[2].y += g.z * ta.y;
-                acc[3].x += g.w * ta.x; acc[3].y += g.w * ta.y;
             }
         }
         float2 H = make_float2(0.f, 0.f);
@@ -216,6 +217,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     __shared__ double2 newv[2];
     __shared__ float2 dlt[2];
     __shared__ int pair[2];
+    __shared__ uint32_t pbc[PAIRS ? (kMaxPairsN - 1) * (kMaxPairsN - 2) / 2 : 1];
 
     const int rep = blockIdx.x;
     const int tid = threadIdx.x, bdim = blockDim.x;
@@ -228,6 +230,8 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         s64[i] = v;
         s32[i] = make_float2((float)v.x, (float)v.y);
     }
+    if (PAIRS)
+        for (int i = tid; i < a.npairs; i += bdim) pbc[i] = a.pair_bc[i];
     const double beta = a.betas[rep];
     const uint64_t seed = a.seeds[rep];
     double E = 0.0;
@@ -313,7 +317,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 local = -(di.x * Fi.x - di.y * Fi.y) - (dj.x * Fj.x - dj.y * Fj.y);
             }
             if (PAIRS)
-                local += field_pairs(a, s32, i0, j0, di, dj, w, lane);
+                local += field_pairs(a, s32, pbc, i0, j0, di, dj, w, lane);
             else
                 local += field_lists(a, s32, i0, j0, di, dj, tid, bdim);
             local = wave_sum(local);

@@ -19,6 +19,7 @@
 #ifndef PSPIN_H
 #define PSPIN_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -183,6 +184,11 @@ int pspin_get_rng_info(const pspin_state* st, uint64_t* seeds, uint64_t* sweep_c
 int pspin_set_profiling(pspin_state* st, int on);
 int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweep_launches, double* exch_ms,
                       long long* n_exch_launches);
+
+/* cudaMemGetInfo / cudaGetDeviceProperties counterpart for the drivers'
+ * memory pre-check and banner (src/parallel_tempering.cu:178-202).  dev < 0
+ * means the current device.  name may be NULL. */
+int pspin_device_info(int dev, size_t* free_bytes, size_t* total_bytes, char* name, int name_len);
 
 /* Kernel selection and coupling storage (for the autotuner and tests). */
 #define PSPIN_KERNEL_AUTO 0
