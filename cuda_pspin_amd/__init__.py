@@ -19,7 +19,9 @@ import numpy as np
 __all__ = ["SimConfig", "PSpin", "load_library", "build", "DENSE", "SPARSE", "STEP_DTYPE", "PSpinError"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpspin.so")
+# CUDA_PSPIN_AMD_LIB selects another build of the same library (e.g. a
+# differently tuned one); the default is the in-tree build.
+LIB_PATH = os.environ.get("CUDA_PSPIN_AMD_LIB", os.path.join(_HERE, "libpspin.so"))
 
 DENSE = 0
 SPARSE = 1

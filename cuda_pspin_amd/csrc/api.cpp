@@ -70,7 +70,7 @@ void free_all(pspin_state* st) {
     dfree(st->d_seeds); dfree(st->d_e2); dfree(st->d_e4);
     dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48);
     dfree(st->d_series); dfree(st->d_series_n); dfree(st->d_log);
-    dfree(st->d_ptab); dfree(st->d_row_off); dfree(st->d_row_len); dfree(st->d_pair_bc);
+    dfree(st->d_ptab); dfree(st->d_pair_bc);
     dfree(st->d_tasks); dfree(st->d_wave_tasks);
     for (auto& pr : st->ev_sweep) { st->ev_free.push_back(pr.first); st->ev_free.push_back(pr.second); }
     for (auto& pr : st->ev_exch) { st->ev_free.push_back(pr.first); st->ev_free.push_back(pr.second); }
@@ -199,14 +199,12 @@ int build_pair_layout(pspin_state* st, const std::vector<double>& g4, const std:
     st->pair_npairs = npairs;
     st->pair_nwaves = nw;
     PS_TRY_HIP(dmalloc(&st->d_ptab, tab.size()));
-    PS_TRY_HIP(dmalloc(&st->d_row_off, nrows));
-    PS_TRY_HIP(dmalloc(&st->d_row_len, nrows));
     PS_TRY_HIP(dmalloc(&st->d_pair_bc, pbc.size()));
     PS_TRY_HIP(dmalloc(&st->d_tasks, flat.size()));
     PS_TRY_HIP(dmalloc(&st->d_wave_tasks, wt.size()));
     PS_TRY_HIP(cpy(st, st->d_ptab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
-    PS_TRY_HIP(cpy(st, st->d_row_off, roff.data(), nrows * sizeof(int), hipMemcpyHostToDevice));
-    PS_TRY_HIP(cpy(st, st->d_row_len, rlen.data(), nrows * sizeof(int), hipMemcpyHostToDevice));
+    st->h_row_off = roff;
+    st->h_row_len = rlen;
     PS_TRY_HIP(cpy(st, st->d_pair_bc, pbc.data(), pbc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     PS_TRY_HIP(cpy(st, st->d_tasks, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
     PS_TRY_HIP(cpy(st, st->d_wave_tasks, wt.data(), wt.size() * sizeof(int), hipMemcpyHostToDevice));

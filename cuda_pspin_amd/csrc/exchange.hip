@@ -39,6 +39,10 @@ __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
         sperm[i] = a.perm[i];
     }
     for (int i = lane; i < NT; i += 64) sB[i] = a.beta_sched[i];
+    // Jump table: k steps of the 48-bit LCG, k = 0..64, as one multiply-add.
+    __shared__ uint64_t jA[65], jC[65];
+    lcg48_coeffs((uint64_t)lane + 1, &jA[lane + 1], &jC[lane + 1]);
+    if (lane == 0) { jA[0] = 1; jC[0] = 0; }
     __syncthreads();
     uint64_t X = *a.rng48;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -54,10 +58,12 @@ __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
             const unsigned long long m = __ballot(need);
             bool accept = valid;
             if (need) {
-                const uint64_t x = lcg48_skip(X, (uint64_t)__popcll(m & lt) + 1);
+                const int k = __popcll(m & lt) + 1;
+                const uint64_t x = (jA[k] * X + jC[k]) & kMask48;
                 accept = lcg48_unit(x) < exp(delta);
             }
-            X = lcg48_skip(X, (uint64_t)__popcll(m));
+            const int tot = __popcll(m);
+            X = (jA[tot] * X + jC[tot]) & kMask48;
             if (valid && accept) {
                 const int tmp = sperm[t * nr + r];
                 sperm[t * nr + r] = sperm[(t + 1) * nr + r];

@@ -103,6 +103,22 @@ PS_HD uint64_t lcg48_skip(uint64_t x, uint64_t k) {
     return (am * x + ap) & kMask48;
 }
 
+// Affine coefficients (A, C) of k steps: X_{n+k} = A X_n + C (mod 2^48).
+PS_HD void lcg48_coeffs(uint64_t k, uint64_t* A, uint64_t* C) {
+    uint64_t am = 1, ap = 0, cm = kLcgA, cp = kLcgC;
+    while (k) {
+        if (k & 1) {
+            am = am * cm;
+            ap = ap * cm + cp;
+        }
+        cp = (cm + 1) * cp;
+        cm = cm * cm;
+        k >>= 1;
+    }
+    *A = am & kMask48;
+    *C = ap & kMask48;
+}
+
 PS_HD double lcg48_unit(uint64_t x) { return (double)x * (1.0 / 281474976710656.0); }
 
 PS_HD long long n_quartets(int N) { return (long long)N * (N - 1) * (N - 2) * (N - 3) / 24; }
@@ -164,8 +180,8 @@ struct pspin_state {
     // dense fully connected per-site pair-row slices (PSPIN_KERNEL_PAIRS)
     float* d_ptab = nullptr;      // [N][pair_slice_len]
     int pair_slice_len = 0;
-    int* d_row_off = nullptr;     // [N-3]
-    int* d_row_len = nullptr;     // [N-3]
+    std::vector<int> h_row_off;   // [N-3]
+    std::vector<int> h_row_len;   // [N-3]
     uint32_t* d_pair_bc = nullptr;
     int pair_npairs = 0;
     int4* d_tasks = nullptr;

@@ -61,8 +61,10 @@ struct SweepArgs {
     // PAIRS layout
     const float* ptab;        // [N][slice_len]
     int slice_len;
-    const int* row_off;       // [N-3] offset of row a in a slice (floats)
-    const int* row_len;       // [N-3] length of row a, multiple of 4
+    // Row tables travel in the kernel arguments (constant memory), so the
+    // per-row reads are scalar loads with no aliasing doubt.
+    int row_off[kMaxPairsN - 3];  // offset of row a in a slice (floats)
+    int row_len[kMaxPairsN - 3];  // length of row a, multiple of 4
     const uint32_t* pair_bc;  // [npairs] b | c << 16
     int npairs;
     const int4* tasks;        // {slot (0: i0, 1: j0), first chunk, a0, a1}
@@ -221,7 +223,9 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
 
     const int rep = blockIdx.x;
     const int tid = threadIdx.x, bdim = blockDim.x;
-    const int nw = bdim >> 6, w = tid >> 6, lane = tid & 63;
+    // The wave index is uniform across the wave; saying so lets the compiler
+    // keep the task and row tables on the scalar path.
+    const int nw = bdim >> 6, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int N = a.N, nsteps = N / 2;
     double* gs = a.spins + (size_t)2 * N * rep;
 
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             __syncthreads();
             if (step == nsteps) break;
 
-            const int i0 = pair[0], j0 = pair[1];
+            const int i0 = __builtin_amdgcn_readfirstlane(pair[0]), j0 = __builtin_amdgcn_readfirstlane(pair[1]);
             const float2 di = dlt[0], dj = dlt[1];
             float local = 0.f;
             if (a.h2_active) {
@@ -358,8 +362,10 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
     a.log_base = st->log_n;
     a.ptab = st->d_ptab;
     a.slice_len = st->pair_slice_len;
-    a.row_off = st->d_row_off;
-    a.row_len = st->d_row_len;
+    for (int r = 0; r < (int)st->h_row_off.size(); r++) {
+        a.row_off[r] = st->h_row_off[r];
+        a.row_len[r] = st->h_row_len[r];
+    }
     a.pair_bc = st->d_pair_bc;
     a.npairs = st->pair_npairs;
     a.tasks = st->d_tasks;
