@@ -16,7 +16,7 @@ import subprocess
 
 import numpy as np
 
-__all__ = ["SimConfig", "PSpin", "load_library", "build", "DENSE", "SPARSE", "STEP_DTYPE", "PSpinError"]
+__all__ = ["SimConfig", "PSpin", "PSpinBatch", "load_library", "build", "DENSE", "SPARSE", "STEP_DTYPE", "PSpinError"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # CUDA_PSPIN_AMD_LIB selects another build of the same library (e.g. a
@@ -134,6 +134,16 @@ def load_library() -> C.CDLL:
         "pspin_get_profile": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
         "pspin_set_kernel": (C.c_int, [P, C.c_int, C.c_int]),
+        "pspin_set_resync_interval": (C.c_int, [P, C.c_int]),
+        "pspin_batch_create": (C.c_int, [P, C.c_int, i32p, u64p, C.c_int, C.c_int, dp, C.POINTER(P)]),
+        "pspin_batch_destroy": (C.c_int, [P]),
+        "pspin_batch_set_resync_interval": (C.c_int, [P, C.c_int]),
+        "pspin_batch_run": (C.c_int, [P, C.c_int, C.c_int]),
+        "pspin_batch_info": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+        "pspin_batch_get_results": (C.c_int, [P, C.c_int, dp, i64p, i64p]),
+        "pspin_batch_get_spins": (C.c_int, [P, C.c_int, dp]),
+        "pspin_batch_get_perm": (C.c_int, [P, C.c_int, i32p]),
+        "pspin_batch_get_exchange_stats": (C.c_int, [P, C.c_int, i64p, i64p]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -320,8 +330,73 @@ class PSpin:
         _check(self._L.pspin_get_profile(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         return {"sweep_ms": a.value, "n_sweep": b.value, "exch_ms": c.value, "n_exch": d.value}
 
+    def set_resync_interval(self, sweeps: int) -> None:
+        _check(self._L.pspin_set_resync_interval(self._h, int(sweeps)))
+
     def set_kernel(self, kernel: int = KERNEL_AUTO, block_size: int = 0) -> None:
         _check(self._L.pspin_set_kernel(self._h, int(kernel), int(block_size)))
         info = Info()
         _check(self._L.pspin_get_info(self._h, C.byref(info)))
         self.info = info
+
+
+class PSpinBatch:
+    """Many independent sparse (smoothed-cube) PT systems of uneven N in one launch.
+
+    Sample s uses N = Ns[s] and seed = seeds[s]; every sample has NT x nrep_per_temp
+    chains on beta_sched.  Mirrors running the reference's parallel_tempering_sparse
+    once per sample (one process per label in scan_samples_pt).
+    """
+
+    def __init__(self, base: SimConfig, Ns, seeds, NT: int, nrep_per_temp: int, beta_sched):
+        self._L = load_library()
+        self._h = C.c_void_p()
+        self.Ns = np.ascontiguousarray(Ns, dtype=np.int32)
+        self.seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        self.NT, self.nrep_per_temp = NT, nrep_per_temp
+        self.chains_per_sample = NT * nrep_per_temp
+        _check(self._L.pspin_batch_create(C.byref(base), len(self.Ns), self.Ns, self.seeds, NT, nrep_per_temp,
+                                          np.ascontiguousarray(beta_sched, dtype=np.float64), C.byref(self._h)))
+        n, ch, pu = C.c_int(), C.c_longlong(), C.c_longlong()
+        _check(self._L.pspin_batch_info(self._h, C.byref(n), C.byref(ch), C.byref(pu)))
+        self.n_samples, self.chains, self.pair_updates_per_sweep = n.value, ch.value, pu.value
+
+    def close(self):
+        if self._h:
+            self._L.pspin_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_resync_interval(self, sweeps: int):
+        _check(self._L.pspin_batch_set_resync_interval(self._h, int(sweeps)))
+
+    def run(self, n_sweeps: int, pt_freq: int = 1):
+        _check(self._L.pspin_batch_run(self._h, int(n_sweeps), int(pt_freq)))
+
+    def results(self, s: int):
+        c = self.chains_per_sample
+        E, a, p = np.zeros(c), np.zeros(c, dtype=np.int64), np.zeros(c, dtype=np.int64)
+        _check(self._L.pspin_batch_get_results(self._h, s, E, a, p))
+        return E, a, p
+
+    def spins(self, s: int):
+        c, N = self.chains_per_sample, int(self.Ns[s])
+        out = np.zeros(2 * N * c)
+        _check(self._L.pspin_batch_get_spins(self._h, s, out))
+        out = out.reshape(c, N, 2)
+        return out[..., 0] + 1j * out[..., 1]
+
+    def perm(self, s: int):
+        p = np.zeros(self.chains_per_sample, dtype=np.int32)
+        _check(self._L.pspin_batch_get_perm(self._h, s, p))
+        return p
+
+    def exchange_stats(self, s: int):
+        a, p = np.zeros(self.NT - 1, dtype=np.int64), np.zeros(self.NT - 1, dtype=np.int64)
+        _check(self._L.pspin_batch_get_exchange_stats(self._h, s, a, p))
+        return a, p

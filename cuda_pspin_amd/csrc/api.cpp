@@ -67,7 +67,9 @@ void free_all(pspin_state* st) {
     dfree(st->d_g2_64); dfree(st->d_g2_32); dfree(st->d_g4_64); dfree(st->d_mask);
     dfree(st->d_sq); dfree(st->d_sqg); dfree(st->d_site_off); dfree(st->d_site_ent);
     dfree(st->d_spins); dfree(st->d_E); dfree(st->d_betas); dfree(st->d_acc); dfree(st->d_prop);
-    dfree(st->d_seeds); dfree(st->d_e2); dfree(st->d_e4);
+    dfree(st->d_seeds); dfree(st->d_e2); dfree(st->d_e4); dfree(st->d_ctr);
+    if (st->graph) (void)hipGraphExecDestroy(st->graph);
+    st->graph = nullptr;
     dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48);
     dfree(st->d_series); dfree(st->d_series_n); dfree(st->d_log);
     dfree(st->d_ptab); dfree(st->d_pair_bc);
@@ -303,6 +305,8 @@ int sweep_launch(pspin_state* st, int n) {
         PS_TRY_HIP(hipEventRecord(e0, st->stream));
     }
     PS_TRY_HIP(ps::launch_sweep(st, n));
+    st->sweep_counter += (uint64_t)n;
+    if (st->d_log) st->log_n = std::min(st->log_cap, st->log_n + n);
     if (st->prof) {
         PS_TRY_HIP(hipEventRecord(e1, st->stream));
         st->ev_sweep.push_back({e0, e1});
@@ -331,6 +335,48 @@ int exchange_launch(pspin_state* st) {
             int rc = harvest_events(st);
             if (rc) return rc;
         }
+    }
+    return PSPIN_OK;
+}
+
+constexpr int kGraphSweeps = 32;  // sweeps per captured graph (rounded to whole PT rounds)
+
+void drop_graph(pspin_state* st) {
+    if (st->graph) (void)hipGraphExecDestroy(st->graph);
+    st->graph = nullptr;
+    st->graph_pt_freq = st->graph_rounds = 0;
+}
+
+// Capture `rounds` rounds of (pt_freq sweeps, one exchange) into a graph.
+// The kernels read their RNG counters and PT state from device memory, so
+// replaying the same graph continues the chains correctly.
+int ensure_graph(pspin_state* st, int pt_freq, int rounds) {
+    if (st->graph && st->graph_pt_freq == pt_freq && st->graph_rounds == rounds) return PSPIN_OK;
+    drop_graph(st);
+    hipGraph_t g = nullptr;
+    PS_TRY_HIP(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < rounds && e == hipSuccess; k++) {
+        e = ps::launch_sweep(st, pt_freq);
+        if (e == hipSuccess) e = ps::launch_exchange(st);
+    }
+    hipError_t e2 = hipStreamEndCapture(st->stream, &g);
+    PS_TRY_HIP(e);
+    PS_TRY_HIP(e2);
+    hipError_t e3 = hipGraphInstantiate(&st->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    PS_TRY_HIP(e3);
+    st->graph_pt_freq = pt_freq;
+    st->graph_rounds = rounds;
+    return PSPIN_OK;
+}
+
+// FP64 recomputation of the running energies every resync_every sweeps.
+int maybe_resync(pspin_state* st) {
+    if (st->resync_every > 0 && st->since_resync >= st->resync_every) {
+        int rc = recompute_energies(st);
+        if (rc) return rc;
+        st->since_resync = 0;
     }
     return PSPIN_OK;
 }
@@ -564,6 +610,8 @@ int pspin_create(const pspin_sim_config* cfg, int variant, pspin_state** out) {
     PS_CREATE_HIP(dmalloc(&st->d_seeds, nrep));
     PS_CREATE_HIP(cpy(st, st->d_seeds, st->h_seeds.data(), nrep * sizeof(uint64_t), hipMemcpyHostToDevice));
     PS_CREATE_HIP(ps::launch_init_spins(st->d_spins, N, nrep, st->d_seeds, variant == PSPIN_SPARSE, st->stream));
+    PS_CREATE_HIP(dmalloc(&st->d_ctr, nrep));
+    PS_CREATE_HIP(hipMemsetAsync(st->d_ctr, 0, nrep * sizeof(uint64_t), st->stream));
     PS_CREATE_HIP(dmalloc(&st->d_E, nrep));
     PS_CREATE_HIP(dmalloc(&st->d_e2, nrep));
     PS_CREATE_HIP(dmalloc(&st->d_e4, nrep));
@@ -622,8 +670,18 @@ int pspin_set_uniform_beta(pspin_state* st, double beta) {
 int pspin_sweep(pspin_state* st, int n_sweeps) {
     PS_REQUIRE(st && n_sweeps >= 0, PSPIN_ERR_ARG, "bad argument");
     if (n_sweeps == 0) return PSPIN_OK;
-    int rc = sweep_launch(st, n_sweeps);
-    if (rc) return rc;
+    int done = 0;
+    while (done < n_sweeps) {
+        // Split at resync points so the FP64 resync lands every resync_every sweeps.
+        int chunk = n_sweeps - done;
+        if (st->resync_every > 0) chunk = (int)std::min<long long>(chunk, st->resync_every - st->since_resync);
+        if (chunk <= 0) chunk = 1;
+        int rc = sweep_launch(st, chunk);
+        if (rc) return rc;
+        done += chunk;
+        st->since_resync += chunk;
+        if ((rc = maybe_resync(st))) return rc;
+    }
     PS_TRY_HIP(hipStreamSynchronize(st->stream));
     return PSPIN_OK;
 }
@@ -731,6 +789,7 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
                "NT * nrep_per_temp must equal nrep and NT >= 2");
     PS_REQUIRE((size_t)NT * nrep_per_temp * 12 + (size_t)NT * 16 <= 60 * 1024, PSPIN_ERR_ARG,
                "too many replicas for the exchange kernel (NT * nrep_per_temp <= 5000)");
+    drop_graph(st);
     dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48);
     st->pt_NT = NT;
     st->pt_nrep = nrep_per_temp;
@@ -758,14 +817,33 @@ int pspin_pt_run(pspin_state* st, int n_sweeps, int pt_freq) {
     PS_REQUIRE(st && n_sweeps >= 0 && pt_freq >= 1, PSPIN_ERR_ARG, "bad argument");
     PS_REQUIRE(st->pt_NT >= 2, PSPIN_ERR_STATE, "pspin_pt_init has not been called");
     int done = 0;
+    // Whole rounds (pt_freq sweeps + one exchange) go through a captured
+    // graph of kGraphRounds rounds when nothing per-launch is being recorded.
+    const bool can_graph = !st->prof && !st->d_log;
+    const int rounds = std::max(1, kGraphSweeps / pt_freq);
     while (done < n_sweeps) {
+        const int left = n_sweeps - done;
+        if (can_graph && st->pt_sweeps % pt_freq == 0 && left >= rounds * pt_freq) {
+            int rc = ensure_graph(st, pt_freq, rounds);
+            if (rc) return rc;
+            PS_TRY_HIP(hipGraphLaunch(st->graph, st->stream));
+            const int nsw = rounds * pt_freq;
+            st->sweep_counter += (uint64_t)nsw;
+            st->pt_sweeps += nsw;
+            done += nsw;
+            st->since_resync += nsw;
+            if ((rc = maybe_resync(st))) return rc;
+            continue;
+        }
         // Sweeps until the next exchange point ((s+1) % pt_freq == 0).
         long long to_next = pt_freq - (st->pt_sweeps % pt_freq);
-        int chunk = (int)std::min<long long>(to_next, n_sweeps - done);
+        int chunk = (int)std::min<long long>(to_next, left);
         int rc = sweep_launch(st, chunk);
         if (rc) return rc;
         st->pt_sweeps += chunk;
         done += chunk;
+        st->since_resync += chunk;
+        if ((rc = maybe_resync(st))) return rc;
         if (st->pt_sweeps % pt_freq == 0) {
             rc = exchange_launch(st);
             if (rc) return rc;
@@ -821,6 +899,7 @@ int pspin_pt_get_rng48(const pspin_state* st, uint64_t* x48) {
 
 int pspin_pt_enable_series(pspin_state* st, int capacity) {
     PS_REQUIRE(st && capacity >= 0 && st->pt_NT >= 2, PSPIN_ERR_STATE, "pspin_pt_init has not been called");
+    drop_graph(st);
     dfree(st->d_series);
     dfree(st->d_series_n);
     st->series_cap = capacity;
@@ -857,6 +936,7 @@ int pspin_pt_get_series(const pspin_state* st, double* out, int* n_out) {
 // ----------------------------- hooks --------------------------------------
 int pspin_enable_step_log(pspin_state* st, int capacity_sweeps) {
     PS_REQUIRE(st && capacity_sweeps >= 0, PSPIN_ERR_ARG, "bad argument");
+    drop_graph(st);
     dfree(st->d_log);
     st->log_cap = capacity_sweeps;
     st->log_n = 0;
@@ -902,6 +982,13 @@ int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweep, dou
     return PSPIN_OK;
 }
 
+int pspin_set_resync_interval(pspin_state* st, int sweeps) {
+    PS_REQUIRE(st && sweeps >= 0, PSPIN_ERR_ARG, "bad argument");
+    st->resync_every = sweeps;
+    st->since_resync = 0;
+    return PSPIN_OK;
+}
+
 int pspin_device_info(int dev, size_t* free_bytes, size_t* total_bytes, char* name, int name_len) {
     int cur = 0;
     PS_TRY_HIP(hipGetDevice(&cur));
@@ -926,6 +1013,7 @@ int pspin_set_kernel(pspin_state* st, int kernel, int block_size) {
                "PSPIN_KERNEL_PAIRS needs a dense fully connected system (fmc_mode 0, 8 <= N <= 128)");
     PS_REQUIRE(block_size == 0 || (block_size % 64 == 0 && block_size >= 64 && block_size <= 1024), PSPIN_ERR_ARG,
                "block_size must be 0 or a multiple of 64 in [64, 1024]");
+    drop_graph(st);
     st->kernel = kernel == PSPIN_KERNEL_AUTO ? (st->d_ptab ? PSPIN_KERNEL_PAIRS : PSPIN_KERNEL_LISTS) : kernel;
     st->block_size = block_size;
     return PSPIN_OK;

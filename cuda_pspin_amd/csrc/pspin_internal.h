@@ -196,7 +196,13 @@ struct pspin_state {
     long long* d_prop = nullptr;
     uint64_t* d_seeds = nullptr;
     std::vector<uint64_t> h_seeds;
-    uint64_t sweep_counter = 0;
+    uint64_t sweep_counter = 0;     // host mirror of d_ctr (all replicas advance together)
+    uint64_t* d_ctr = nullptr;      // [nrep] per-replica sweep counters
+    int resync_every = 1024;        // FP64 energy resync interval in sweeps (0 = never)
+    long long since_resync = 0;
+    // captured sweep+exchange rounds
+    hipGraphExec_t graph = nullptr;
+    int graph_pt_freq = 0, graph_rounds = 0;
     double* d_e2 = nullptr;
     double* d_e4 = nullptr;
 

@@ -54,7 +54,7 @@ struct SweepArgs {
     long long* acc;
     long long* prop;
     const uint64_t* seeds;
-    uint64_t sweep0;
+    uint64_t* ctr;  // [nrep] per-replica sweep counter (Philox counter), advanced by the kernel
     StepRec* log;
     int log_cap;   // sweeps per replica in the log
     int log_base;  // sweeps already logged
@@ -238,12 +238,13 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         for (int i = tid; i < a.npairs; i += bdim) pbc[i] = a.pair_bc[i];
     const double beta = a.betas[rep];
     const uint64_t seed = a.seeds[rep];
+    const uint64_t sweep0 = a.ctr[rep];
     double E = 0.0;
     long long nacc = 0, nprop = 0;
     if (tid == 0) E = a.E[rep];
 
     for (int sw = 0; sw < a.n_sweeps; sw++) {
-        const uint64_t sweep = a.sweep0 + (uint64_t)sw;
+        const uint64_t sweep = sweep0 + (uint64_t)sw;
         for (int k = tid; k < nsteps; k += bdim) pre[k] = make_step<CUBE>(seed, sweep, k, N, beta);
         __syncthreads();
         StepRec* logp = nullptr;
@@ -338,6 +339,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         a.E[rep] = E;
         a.acc[rep] += nacc;
         a.prop[rep] += nprop;
+        a.ctr[rep] = sweep0 + (uint64_t)a.n_sweeps;
     }
 }
 
@@ -356,7 +358,7 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
     a.acc = st->d_acc;
     a.prop = st->d_prop;
     a.seeds = st->d_seeds;
-    a.sweep0 = st->sweep_counter;
+    a.ctr = st->d_ctr;
     a.log = st->d_log;
     a.log_cap = st->log_cap;
     a.log_base = st->log_n;
@@ -383,12 +385,7 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
         hipLaunchKernelGGL((k_sweep<false, true>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
     else
         hipLaunchKernelGGL((k_sweep<false, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) {
-        st->sweep_counter += (uint64_t)n_sweeps;
-        if (st->d_log) st->log_n = std::min(st->log_cap, st->log_n + n_sweeps);
-    }
-    return e;
+    return hipGetLastError();
 }
 
 }  // namespace ps

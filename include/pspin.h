@@ -120,6 +120,11 @@ int pspin_split_energies(const pspin_state* st, double* e2, double* e4);
 /* compute_energy (include/hamiltonian.h:11-17) for all replicas, written
  * into the running energies: the FP64 resync of the running accumulator. */
 int pspin_resync_energy(pspin_state* st);
+
+/* Periodic FP64 resync inside pspin_sweep / pspin_pt_run: every `sweeps`
+ * sweeps (default 1024; 0 = never, as in the reference, which never resyncs
+ * its FP64 running energy, src/mc.cu:301). */
+int pspin_set_resync_interval(pspin_state* st, int sweeps);
 int pspin_set_energies(pspin_state* st, const double* E); /* test hook */
 
 /* Disorder read-back, for tests and the drivers' output files
@@ -167,6 +172,29 @@ int pspin_pt_get_rng48(const pspin_state* st, uint64_t* x48);
  * layout [round][nrep_per_temp]; *n_out receives the number of rounds. */
 int pspin_pt_enable_series(pspin_state* st, int capacity);
 int pspin_pt_get_series(const pspin_state* st, double* out, int* n_out);
+
+/* ---------------- batches of sparse PT systems --------------------------
+ * The reference runs one disorder sample per process (scan_samples_pt loops
+ * over labels).  A batch holds n_samples independent PSPIN_SPARSE systems,
+ * sample s with N = Ns[s] and seed = seeds[s] (other fields from `base`), each
+ * with NT x nrep_per_temp chains on the geometric/linear schedule beta_sched,
+ * and runs them all in one launch: one workgroup per sample, one lane per
+ * chain, the exchange of src/parallel_tempering.cu:316-352 per sample with
+ * srand48(seed + 5000).  Sample s starts from the same disorder, spins and
+ * RNG seeds as pspin_create(.., PSPIN_SPARSE) with that seed. */
+typedef struct pspin_batch pspin_batch;
+int pspin_batch_create(const pspin_sim_config* base, int n_samples, const int* Ns, const uint64_t* seeds, int NT,
+                       int nrep_per_temp, const double* beta_sched, pspin_batch** out);
+int pspin_batch_destroy(pspin_batch* b);
+int pspin_batch_set_resync_interval(pspin_batch* b, int sweeps);
+/* n_sweeps sweeps; exchange after each sweep completing a multiple of pt_freq (0: none). */
+int pspin_batch_run(pspin_batch* b, int n_sweeps, int pt_freq);
+int pspin_batch_info(const pspin_batch* b, int* n_samples, long long* chains, long long* pair_updates_per_sweep);
+/* Per-sample read-back; chains are ordered t * nrep_per_temp + r (physical). */
+int pspin_batch_get_results(const pspin_batch* b, int sample, double* E, long long* acc, long long* prop);
+int pspin_batch_get_spins(const pspin_batch* b, int sample, double* re_im);
+int pspin_batch_get_perm(const pspin_batch* b, int sample, int* perm);
+int pspin_batch_get_exchange_stats(const pspin_batch* b, int sample, long long* acc_tot, long long* prop_tot);
 
 /* ---------------- test and profiling hooks (no reference counterpart) --- */
 /* Per-step log of the next sweeps: capacity_sweeps sweeps per replica.

@@ -88,3 +88,45 @@ def test_pt_run_counts_and_betas(pspin, oracle):
     perm = st.pt_perm()
     assert sorted(perm.tolist()) == list(range(NT * nr))
     st.close()
+
+
+def test_graph_rounds_match_direct_launches(pspin, oracle):
+    # pt_run replays captured (sweep, exchange) rounds when nothing is being
+    # recorded per launch; with profiling on it launches directly.  Both must
+    # give the same chains: the kernels keep their RNG counters on the device.
+    NT, nr = 4, 2
+    T, beta = oracle.temperatures(NT, 0.4, 1.6, 1)
+    out = []
+    for prof in (False, True):
+        cfg = pspin.SimConfig.default(N=16, nrep=NT * nr, seed=321)
+        st = pspin.PSpin(cfg, 0)
+        st.pt_init(NT, nr, beta, 321 + 5000)
+        st.set_profiling(prof)
+        st.pt_run(96, 1)
+        st.pt_run(40, 3)
+        E, acc, prop = st.results()
+        out.append((st.spins(), E, acc, st.pt_perm(), st.pt_rng48(), st.rng_info()[1]))
+        st.close()
+    a, b = out
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[2], b[2])
+    np.testing.assert_array_equal(a[3], b[3])
+    assert a[4] == b[4] and a[5] == b[5] == 136
+
+
+def test_periodic_resync(pspin, oracle):
+    cfg = pspin.SimConfig.default(N=14, nrep=4, seed=77)
+    st = pspin.PSpin(cfg, 0)
+    st.set_resync_interval(16)
+    st.set_uniform_beta(1.3)
+    st.sweep(48)  # resyncs after sweeps 16, 32, 48
+    E, _, _ = st.results()
+    e2, e4 = st.split_energies()
+    np.testing.assert_array_equal(E, e2 + e4)
+    st.set_resync_interval(0)
+    st.sweep(5)
+    E, _, _ = st.results()
+    e2, e4 = st.split_energies()
+    assert np.all(np.abs(E - (e2 + e4)) < 1e-4) and np.any(E != e2 + e4)
+    st.close()
