@@ -119,6 +119,10 @@ def copy_calibration(nbytes):
 
 def load_traffic(N, fmc, kernel):
     """HBM bytes per sweep launch measured with rocprofv3 --pmc (profiles/), or None."""
+    return load_traffic_key(f"N{N}_fmc{fmc}_k{kernel}")
+
+
+def load_traffic_key(key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -126,7 +130,6 @@ def load_traffic(N, fmc, kernel):
         rec = json.load(open(path))
     except ValueError:
         return None
-    key = f"N{N}_fmc{fmc}_k{kernel}"
     return rec.get(key, {}).get("hbm_bytes_per_launch")
 
 
@@ -171,7 +174,7 @@ def run_batch(args):
                       "samples": n_samples, "chains": b.chains, "N_hist": {str(n): Ns.count(n) for n in (24, 32, 48, 64)}},
            "init_s": t_init,
            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK, "traffic": None,
+                        "frac": achieved / HBM_PEAK, "traffic": load_traffic_key("batch_256"),
                         "note": "terms and amplitudes are LDS-resident; latency-bound per lane"}}
     b.close()
     print(json.dumps(out))
