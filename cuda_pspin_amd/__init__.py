@@ -135,6 +135,7 @@ def load_library() -> C.CDLL:
                                         C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
         "pspin_set_kernel": (C.c_int, [P, C.c_int, C.c_int]),
         "pspin_set_resync_interval": (C.c_int, [P, C.c_int]),
+        "pspin_set_coupling_format": (C.c_int, [P, C.c_int]),
         "pspin_batch_create": (C.c_int, [P, C.c_int, i32p, u64p, C.c_int, C.c_int, dp, C.POINTER(P)]),
         "pspin_batch_destroy": (C.c_int, [P]),
         "pspin_batch_set_resync_interval": (C.c_int, [P, C.c_int]),
@@ -329,6 +330,13 @@ class PSpin:
         b, d = C.c_longlong(), C.c_longlong()
         _check(self._L.pspin_get_profile(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         return {"sweep_ms": a.value, "n_sweep": b.value, "exch_ms": c.value, "n_exch": d.value}
+
+    def set_coupling_format(self, fmt: int) -> None:
+        """0 = FP32, 1 = BF16 hi/lo pairs (list kernel only)."""
+        _check(self._L.pspin_set_coupling_format(self._h, int(fmt)))
+        info = Info()
+        _check(self._L.pspin_get_info(self._h, C.byref(info)))
+        self.info = info
 
     def set_resync_interval(self, sweeps: int) -> None:
         _check(self._L.pspin_set_resync_interval(self._h, int(sweeps)))

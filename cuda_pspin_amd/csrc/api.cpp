@@ -124,6 +124,7 @@ int build_site_lists(pspin_state* st, const std::vector<Term>& terms) {
         }
     }
     st->n_ent = (long long)ent.size();
+    st->h_site_ent = ent;
     st->max_site_len = 0;
     for (int i = 0; i < N; i++) st->max_site_len = std::max(st->max_site_len, cnt[i + 1] - cnt[i]);
     PS_TRY_HIP(dmalloc(&st->d_site_off, N + 1));
@@ -649,7 +650,7 @@ int pspin_get_info(const pspin_state* st, pspin_info* info) {
     info->n_quart_active = st->n_quart_active;
     info->n_quartets = st->n_sq;
     info->n_site_entries = st->n_ent;
-    info->coupling_format = PSPIN_COUPLING_F32;
+    info->coupling_format = st->coupling_format;
     info->kernel = st->kernel;
     return PSPIN_OK;
 }
@@ -982,6 +983,39 @@ int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweep, dou
     return PSPIN_OK;
 }
 
+// BF16 hi/lo split of an FP32 coupling: hi = bf16(g) (round to nearest even),
+// lo = bf16(g - hi); packed hi | lo << 16.
+static uint32_t bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t lsb = (u >> 16) & 1u;
+    return (u + 0x7fffu + lsb) >> 16;
+}
+
+int pspin_set_coupling_format(pspin_state* st, int format) {
+    PS_REQUIRE(st, PSPIN_ERR_ARG, "null argument");
+    PS_REQUIRE(format == PSPIN_COUPLING_F32 || format == PSPIN_COUPLING_BF16X2, PSPIN_ERR_ARG, "unknown format");
+    PS_REQUIRE(format == PSPIN_COUPLING_F32 || st->kernel == PSPIN_KERNEL_LISTS, PSPIN_ERR_STATE,
+               "BF16 hi/lo couplings are served by the list kernel (pspin_set_kernel(.., PSPIN_KERNEL_LISTS, ..))");
+    std::vector<uint2> ent = st->h_site_ent;
+    if (format == PSPIN_COUPLING_BF16X2)
+        for (uint2& e : ent) {
+            float g;
+            std::memcpy(&g, &e.y, 4);
+            const uint32_t hi = bf16_rne(g);
+            float hf;
+            const uint32_t hb = hi << 16;
+            std::memcpy(&hf, &hb, 4);
+            const uint32_t lo = bf16_rne(g - hf);
+            e.y = hi | (lo << 16);
+        }
+    if (!ent.empty())
+        PS_TRY_HIP(cpy(st, st->d_site_ent, ent.data(), ent.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    drop_graph(st);
+    st->coupling_format = format;
+    return PSPIN_OK;
+}
+
 int pspin_set_resync_interval(pspin_state* st, int sweeps) {
     PS_REQUIRE(st && sweeps >= 0, PSPIN_ERR_ARG, "bad argument");
     st->resync_every = sweeps;
@@ -1013,8 +1047,11 @@ int pspin_set_kernel(pspin_state* st, int kernel, int block_size) {
                "PSPIN_KERNEL_PAIRS needs a dense fully connected system (fmc_mode 0, 8 <= N <= 128)");
     PS_REQUIRE(block_size == 0 || (block_size % 64 == 0 && block_size >= 64 && block_size <= 1024), PSPIN_ERR_ARG,
                "block_size must be 0 or a multiple of 64 in [64, 1024]");
+    const int k = kernel == PSPIN_KERNEL_AUTO ? (st->d_ptab ? PSPIN_KERNEL_PAIRS : PSPIN_KERNEL_LISTS) : kernel;
+    PS_REQUIRE(k == PSPIN_KERNEL_LISTS || st->coupling_format == PSPIN_COUPLING_F32, PSPIN_ERR_STATE,
+               "the pair-row kernel stores FP32 couplings; set PSPIN_COUPLING_F32 first");
     drop_graph(st);
-    st->kernel = kernel == PSPIN_KERNEL_AUTO ? (st->d_ptab ? PSPIN_KERNEL_PAIRS : PSPIN_KERNEL_LISTS) : kernel;
+    st->kernel = k;
     st->block_size = block_size;
     return PSPIN_OK;
 }

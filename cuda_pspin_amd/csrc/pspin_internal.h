@@ -176,6 +176,8 @@ struct pspin_state {
     uint2* d_site_ent = nullptr;    // [n_ent]
     long long n_ent = 0;
     int max_site_len = 0;
+    std::vector<uint2> h_site_ent;  // FP32 entries, kept to re-encode the coupling words
+    int coupling_format = PSPIN_COUPLING_F32;
 
     // dense fully connected per-site pair-row slices (PSPIN_KERNEL_PAIRS)
     float* d_ptab = nullptr;      // [N][pair_slice_len]

@@ -116,6 +116,16 @@ __device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t sweep, int 
 // independent 8-byte loads before using any, so enough bytes are in flight
 // to cover L2/MALL latency.  Entries past the end read as g = 0.
 constexpr int kListBatch = 8;
+
+// Coupling word of a list entry: FP32, or a BF16 hi/lo pair (hi in the low
+// half) whose sum carries ~16 mantissa bits; accumulation stays FP32.
+template <bool BF16>
+__device__ __forceinline__ float coupling(uint32_t w) {
+    if (!BF16) return __uint_as_float(w);
+    return __uint_as_float(w << 16) + __uint_as_float(w & 0xffff0000u);
+}
+
+template <bool BF16>
 __device__ __forceinline__ float field_lists(const SweepArgs& a, const float2* s32, int i0, int j0, float2 di,
                                              float2 dj, int tid, int bdim) {
     float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
@@ -132,7 +142,7 @@ __device__ __forceinline__ float field_lists(const SweepArgs& a, const float2* s
 #pragma unroll
         for (int u = 0; u < kListBatch; u++) {
             const float2 tp = s32[en[u].x & 0xff], tm = s32[(en[u].x >> 8) & 0xff], tn = s32[(en[u].x >> 16) & 0xff];
-            const float g = __uint_as_float(en[u].y);
+            const float g = coupling<BF16>(en[u].y);
             const float2 mn = cmulf(tm, tn);
             // g X with X = tp * conj(tm tn)
             const float xr = g * (tp.x * mn.x + tp.y * mn.y), xi = g * (tp.y * mn.x - tp.x * mn.y);
@@ -210,7 +220,7 @@ __device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s
     return local;
 }
 
-template <bool CUBE, bool PAIRS>
+template <bool CUBE, bool PAIRS, bool BF16>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     __shared__ double2 s64[kMaxN];
     __shared__ float2 s32[kMaxN];
@@ -324,7 +334,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             if (PAIRS)
                 local += field_pairs(a, s32, pbc, i0, j0, di, dj, w, lane);
             else
-                local += field_lists(a, s32, i0, j0, di, dj, tid, bdim);
+                local += field_lists<BF16>(a, s32, i0, j0, di, dj, tid, bdim);
             local = wave_sum(local);
             if (lane == 0) part[step & 1][w] = local;
             __syncthreads();
@@ -379,12 +389,16 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
         long long per_step = 2LL * st->max_site_len;
         bs = per_step >= 8192 ? 1024 : per_step >= 1024 ? 256 : 64;
     }
-    if (st->variant == PSPIN_SPARSE)
-        hipLaunchKernelGGL((k_sweep<true, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
-    else if (pairs)
-        hipLaunchKernelGGL((k_sweep<false, true>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
-    else
-        hipLaunchKernelGGL((k_sweep<false, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+    const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
+    if (st->variant == PSPIN_SPARSE) {
+        if (bf16) hipLaunchKernelGGL((k_sweep<true, false, true>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        else hipLaunchKernelGGL((k_sweep<true, false, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+    } else if (pairs) {
+        hipLaunchKernelGGL((k_sweep<false, true, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+    } else {
+        if (bf16) hipLaunchKernelGGL((k_sweep<false, false, true>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        else hipLaunchKernelGGL((k_sweep<false, false, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+    }
     return hipGetLastError();
 }
 

@@ -225,6 +225,9 @@ int pspin_device_info(int dev, size_t* free_bytes, size_t* total_bytes, char* na
 #define PSPIN_COUPLING_F32 0
 #define PSPIN_COUPLING_BF16X2 1 /* BF16 hi/lo split, FP32 accumulation       */
 int pspin_set_kernel(pspin_state* st, int kernel, int block_size);
+/* Coupling storage of the list kernel: FP32 (default) or BF16 hi/lo pairs
+ * (g ~ hi + lo, ~16 mantissa bits) with FP32 accumulation. */
+int pspin_set_coupling_format(pspin_state* st, int format);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,7 @@ def replay_check(O, st, n_sweeps, betas):
     E_gpu, acc, prop = st.results()
     worst = 0.0
     worst_rel = 0.0
+    errs = []
     n_acc = 0
     for r in range(nrep):
         s = s0[r].copy()
@@ -82,6 +83,7 @@ def replay_check(O, st, n_sweeps, betas):
                 err = abs(float(rec["dE"]) - dE64)
                 assert err <= DE_TOL * scale + 1e-30, (r, sw, k, float(rec["dE"]), dE64, scale)
                 worst = max(worst, err / max(scale, 1e-300))
+                errs.append(err)
                 worst_rel = max(worst_rel, err / max(abs(dE64), 1e-3 * scale, 1e-300))
                 thr = math.inf if beta == 0.0 else -math.log(u[5]) / beta
                 want = beta == 0.0 or float(rec["dE"]) < thr
@@ -98,32 +100,40 @@ def replay_check(O, st, n_sweeps, betas):
         assert abs(E_gpu[r] - E_full) / N <= 1e-5, (E_gpu[r], E_full)
     assert int(prop.sum()) == nrep * n_sweeps * steps
     assert int(acc.sum()) == n_acc
-    return dict(worst_scaled=worst, worst_rel=worst_rel, acc_rate=n_acc / (nrep * n_sweeps * steps))
+    return dict(worst_scaled=worst, worst_rel=worst_rel, acc_rate=n_acc / (nrep * n_sweeps * steps),
+                max_abs=float(np.max(errs)), rms_abs=float(np.sqrt(np.mean(np.square(errs)))))
 
 
-@pytest.mark.parametrize("name,variant,kernel,kw,T,nsw", [
-    ("dense FC N=12 pairs", 0, 2, dict(N=12, fmc_mode=0), 1.0, 6),
-    ("dense FC N=12 lists", 0, 1, dict(N=12, fmc_mode=0), 1.0, 6),
-    ("dense uniform N=32 (config-1 system)", 0, 1, dict(N=32, fmc_mode=2, gamma=0.05), 0.4, 3),
-    ("dense comb N=16", 0, 1, dict(N=16, fmc_mode=1, gamma=1.0), 0.8, 4),
-    ("dense FC N=20 J0 pairs", 0, 2, dict(N=20, fmc_mode=0, J0=0.4, alpha0=0.5), 0.6, 2),
-    ("dense FC N=64 pairs (bench system)", 0, 2, dict(N=64, fmc_mode=0), 0.3, 2),
-    ("dense FC N=64 lists", 0, 1, dict(N=64, fmc_mode=0), 0.3, 1),
-    ("dense uniform N=64 (learned-proposal system)", 0, 1, dict(N=64, fmc_mode=2, gamma=0.05), 0.3, 1),
-    ("dense FC N=33 odd pairs", 0, 2, dict(N=33, fmc_mode=0), 0.7, 2),
-    ("sparse cube N=16", 1, 1, dict(N=16, fmc_mode=2, gamma=0.05), 0.5, 10),
-    ("sparse cube FC N=24", 1, 1, dict(N=24, fmc_mode=0), 1.0, 6),
+@pytest.mark.parametrize("name,variant,kernel,kw,T,nsw,fmt", [
+    ("dense FC N=12 pairs", 0, 2, dict(N=12, fmc_mode=0), 1.0, 6, 0),
+    ("dense FC N=12 lists", 0, 1, dict(N=12, fmc_mode=0), 1.0, 6, 0),
+    ("dense uniform N=32 (config-1 system)", 0, 1, dict(N=32, fmc_mode=2, gamma=0.05), 0.4, 3, 0),
+    ("dense comb N=16", 0, 1, dict(N=16, fmc_mode=1, gamma=1.0), 0.8, 4, 0),
+    ("dense FC N=20 J0 pairs", 0, 2, dict(N=20, fmc_mode=0, J0=0.4, alpha0=0.5), 0.6, 2, 0),
+    ("dense FC N=64 pairs (bench system)", 0, 2, dict(N=64, fmc_mode=0), 0.3, 2, 0),
+    ("dense FC N=64 lists", 0, 1, dict(N=64, fmc_mode=0), 0.3, 1, 0),
+    ("dense uniform N=64 (learned-proposal system)", 0, 1, dict(N=64, fmc_mode=2, gamma=0.05), 0.3, 1, 0),
+    ("dense FC N=33 odd pairs", 0, 2, dict(N=33, fmc_mode=0), 0.7, 2, 0),
+    ("sparse cube N=16", 1, 1, dict(N=16, fmc_mode=2, gamma=0.05), 0.5, 10, 0),
+    ("sparse cube FC N=24", 1, 1, dict(N=24, fmc_mode=0), 1.0, 6, 0),
+    ("dense comb N=128 BF16 hi/lo (config-4 system)", 0, 1, dict(N=128, fmc_mode=1, gamma=1.0), 0.3, 1, 1),
+    ("dense comb N=128 FP32", 0, 1, dict(N=128, fmc_mode=1, gamma=1.0), 0.3, 1, 0),
+    ("dense uniform N=32 BF16 hi/lo", 0, 1, dict(N=32, fmc_mode=2, gamma=0.05), 0.4, 3, 1),
+    ("sparse cube N=16 BF16 hi/lo", 1, 1, dict(N=16, fmc_mode=2, gamma=0.05), 0.5, 10, 1),
 ])
-def test_step_replay(pspin, oracle, name, variant, kernel, kw, T, nsw):
+def test_step_replay(pspin, oracle, name, variant, kernel, kw, T, nsw, fmt):
     print(f"[step replay] {name}", flush=True)
     cfg = pspin.SimConfig.default(nrep=4, seed=1234, **kw)
     st = pspin.PSpin(cfg, variant)
     st.set_kernel(kernel, 0)
     assert st.info.kernel == kernel
+    st.set_coupling_format(fmt)
+    assert st.info.coupling_format == fmt
     betas = np.array([1.0 / T, 2.0 / T, 0.5 / T, 0.0])
     stats = replay_check(oracle, st, nsw, betas)
     print(f"  max |dE32-dE64|/S = {stats['worst_scaled']:.2e}  vs |dE64|: {stats['worst_rel']:.2e}  "
-          f"acc={stats['acc_rate']:.3f}", flush=True)
+          f"max |dE-dE64| = {stats['max_abs']:.2e}  rms = {stats['rms_abs']:.2e}  acc={stats['acc_rate']:.3f}",
+          flush=True)
     st.close()
 
 
