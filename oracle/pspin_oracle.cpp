@@ -642,6 +642,86 @@ double or_cpu_baseline_sweeps(int N, int nrep, double* spins, const double* g2, 
     return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// ---------------------------------------------------------------------------
+// CPU parallel tempering with the reference's driver logic
+// (src/parallel_tempering.cu:273-352): FP64 sweeps (ref_sweep_one, the
+// reference's 3-type enumeration), exchange after every sweep in hot-to-cold
+// order with a POSIX drand48 stream seeded (seed48 << 16 | 0x330E) kept
+// locally (thread-safe, the same recurrence glibc uses).
+// Measures, from sweep `measure_from` on, per temperature slot t:
+//   e_sum[t], e2_sum[t]  (E/N, (E/N)^2 summed over replicas r of slot t)
+//   q2_sum[t]            (|q|^2, q = sum_k a_k^(r0) conj(a_k^(r1)) / N, replicas 0,1 of the slot)
+//   acc[t], prop[t]      (Metropolis acceptances / proposals made at slot t)
+// and ex_acc[t], ex_prop[t] for t < NT-1.  *n_meas = measured sweeps.
+// ---------------------------------------------------------------------------
+static double lcg48_next(uint64_t* x) {
+    *x = (0x5DEECE66DULL * *x + 0xBULL) & ((1ULL << 48) - 1);
+    return (double)*x / 281474976710656.0;
+}
+
+void or_cpu_pt(int N, int NT, int nr, double* spins, const double* g2, int h2_active, const double* g4,
+               const uint8_t* mask, const double* beta_sched, const uint64_t* rseeds, int n_sweeps, long seed48,
+               int measure_from, double* e_sum, double* e2_sum, double* q2_sum, long long* acc, long long* prop,
+               long long* ex_acc, long long* ex_prop, int* n_meas) {
+    const int R = NT * nr;
+    std::vector<int> perm(R), slot(R);
+    for (int i = 0; i < R; i++) perm[i] = i;
+    std::vector<double> E(R);
+    for (int r = 0; r < R; r++) {
+        double e2, e4;
+        or_energy(N, spins + (size_t)2 * N * r, g2, h2_active, g4, mask, &e2, &e4);
+        E[r] = e2 + e4;
+    }
+    uint64_t X = (((uint64_t)(uint32_t)seed48) << 16) | 0x330EULL;
+    *n_meas = 0;
+    for (int sw = 0; sw < n_sweeps; sw++) {
+        for (int t = 0; t < NT; t++)
+            for (int k = 0; k < nr; k++) slot[perm[t * nr + k]] = t;
+        for (int r = 0; r < R; r++) {
+            long long a = 0;
+            const int t = slot[r];
+            ref_sweep_one(N, spins + (size_t)2 * N * r, g2, h2_active, g4, mask, beta_sched[t], rseeds[r],
+                          (uint64_t)sw, &E[r], &a);
+            if (sw >= measure_from) {
+                acc[t] += a;
+                prop[t] += N / 2;
+            }
+        }
+        for (int t = 0; t < NT - 1; t++)
+            for (int k = 0; k < nr; k++) {
+                const int pa = perm[t * nr + k], pb = perm[(t + 1) * nr + k];
+                const double d = (beta_sched[t] - beta_sched[t + 1]) * (E[pa] - E[pb]);
+                bool ok = d >= 0.0;
+                if (!ok) ok = lcg48_next(&X) < std::exp(d);
+                if (sw >= measure_from) ex_prop[t]++;
+                if (ok) {
+                    if (sw >= measure_from) ex_acc[t]++;
+                    std::swap(perm[t * nr + k], perm[(t + 1) * nr + k]);
+                }
+            }
+        if (sw >= measure_from) {
+            for (int t = 0; t < NT; t++) {
+                for (int k = 0; k < nr; k++) {
+                    const double e = E[perm[t * nr + k]] / N;
+                    e_sum[t] += e;
+                    e2_sum[t] += e * e;
+                }
+                if (nr >= 2) {
+                    const double* a0 = spins + (size_t)2 * N * perm[t * nr];
+                    const double* a1 = spins + (size_t)2 * N * perm[t * nr + 1];
+                    double qr = 0, qi = 0;
+                    for (int i = 0; i < N; i++) {
+                        qr += a0[2 * i] * a1[2 * i] + a0[2 * i + 1] * a1[2 * i + 1];
+                        qi += a0[2 * i + 1] * a1[2 * i] - a0[2 * i] * a1[2 * i + 1];
+                    }
+                    q2_sum[t] += (qr * qr + qi * qi) / ((double)N * N);
+                }
+            }
+            (*n_meas)++;
+        }
+    }
+}
+
 int or_max_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();
