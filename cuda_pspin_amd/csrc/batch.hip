@@ -81,8 +81,8 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
         if (live) {
             const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
             for (int step = 0; step < nsteps; step++) {
-                const U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 0u}, k0, k1);
-                const U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 1u}, k0, k1);
+                const U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep0}, k0, k1);
+                const U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep1}, k0, k1);
                 // pair pick (src/mc_sparse.cu:118-126)
                 int ri = (int)(u32_to_unit(x.x) * N);
                 if (ri >= N) ri = N - 1;

@@ -22,6 +22,11 @@ constexpr int kWave = 64;          // CDNA wavefront width
 constexpr uint32_t kTagSpins = 1;  // Philox stream tags (second key word is the seed)
 constexpr uint32_t kTagG2 = 2;
 constexpr uint32_t kTagG4 = 4;
+// Fourth Philox counter word of the per-step sweep draws (two blocks per
+// step).  Distinct from the tags above, which share the replica key with
+// spin initialisation, so no sweep draw reuses an initialisation block.
+constexpr uint32_t kCtrStep0 = 16;
+constexpr uint32_t kCtrStep1 = 17;
 
 // ---------------------------------------------------------------------------
 // splitmix64 (Steele, Lea, Flood 2014) -- the reference's seed chain

@@ -86,8 +86,8 @@ __device__ __forceinline__ float wave_sum(float v) {
 template <bool CUBE>
 __device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t sweep, int step, int N, double beta) {
     uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-    U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 0u}, k0, k1);
-    U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 1u}, k0, k1);
+    U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep0}, k0, k1);
+    U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep1}, k0, k1);
     StepPre p;
     double u1 = u32_to_unit(x.x), u2 = u32_to_unit(x.y);
     int ri = (int)(u1 * N);

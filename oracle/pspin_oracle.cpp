@@ -524,8 +524,8 @@ double or_normal(uint64_t seed, uint32_t tag, uint64_t k) {
 // Metropolis u.
 void or_step_uniforms(uint64_t rseed, uint64_t sweep, int step, double* u) {
     uint32_t key[2] = {(uint32_t)rseed, (uint32_t)(rseed >> 32)};
-    uint32_t c0[4] = {(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 0u};
-    uint32_t c1[4] = {(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 1u};
+    uint32_t c0[4] = {(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 16u};
+    uint32_t c1[4] = {(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 17u};
     uint32_t x[4], y[4];
     or_philox4x32_10(c0, key, x);
     or_philox4x32_10(c1, key, y);

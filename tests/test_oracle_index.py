@@ -82,3 +82,15 @@ def test_three_type_order_matches_reference_t_index(oracle):
     assert all(i0 in r and j0 not in r for r in got[:n12])
     assert all(j0 in r and i0 not in r for r in got[n12:2 * n12])
     assert all(i0 in r and j0 in r for r in got[2 * n12:])
+
+
+def test_sweep_draws_do_not_reuse_spin_init_blocks(oracle):
+    """Spin initialisation draws Philox blocks (pair, 0, 0, 1) under the
+    replica key; the sweep's per-step blocks must never coincide with them."""
+    seed = 0x243F6A8885A308D3
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    for s in range(16):
+        init = np.array([(int(x) + 0.5) / 2**32 for x in oracle.philox([s, 0, 0, 1], key)])
+        u = oracle.step_uniforms(seed, s, 0)
+        assert not np.allclose(u[:4], init)
+        assert not np.allclose(u[4:6], init[:2])
