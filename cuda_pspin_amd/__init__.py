@@ -97,6 +97,7 @@ def load_library() -> C.CDLL:
     i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
     u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
     u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+    f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     sigs = {
         "pspin_last_error": (C.c_char_p, []),
         "pspin_default_config": (None, [P]),
@@ -145,6 +146,27 @@ def load_library() -> C.CDLL:
         "pspin_batch_get_spins": (C.c_int, [P, C.c_int, dp]),
         "pspin_batch_get_perm": (C.c_int, [P, C.c_int, i32p]),
         "pspin_batch_get_exchange_stats": (C.c_int, [P, C.c_int, i64p, i64p]),
+        "pspin_lp_default_config": (C.c_int, [P]),
+        "pspin_lp_create": (C.c_int, [P, P, C.POINTER(P)]),
+        "pspin_lp_destroy": (C.c_int, [P]),
+        "pspin_lp_n_params": (C.c_longlong, [P]),
+        "pspin_lp_dims": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]),
+        "pspin_lp_get_params": (C.c_int, [P, f32p]),
+        "pspin_lp_set_params": (C.c_int, [P, f32p]),
+        "pspin_lp_set_scale": (C.c_int, [P, C.c_double]),
+        "pspin_lp_forward": (C.c_int, [P, f32p, C.c_int, f32p]),
+        "pspin_lp_loss_grad": (C.c_int, [P, f32p, f32p, f32p, C.c_int, C.POINTER(C.c_double), f32p]),
+        "pspin_lp_record": (C.c_int, [P]),
+        "pspin_lp_collect": (C.c_int, [P, C.c_longlong, C.c_int, C.c_int]),
+        "pspin_lp_data_size": (C.c_longlong, [P]),
+        "pspin_lp_train": (C.c_int, [P, C.c_longlong, C.c_int, C.c_int, C.c_double, P]),
+        "pspin_lp_move": (C.c_int, [P]),
+        "pspin_lp_run": (C.c_int, [P, C.c_longlong, C.c_int, C.c_int]),
+        "pspin_lp_get_stats": (C.c_int, [P, i64p, i64p]),
+        "pspin_lp_reset_stats": (C.c_int, [P]),
+        "pspin_lp_get_last": (C.c_int, [P, P, P, P, P]),
+        "pspin_lp_get_move_counters": (C.c_int, [P, u64p]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -408,3 +430,118 @@ class PSpinBatch:
         a, p = np.zeros(self.NT - 1, dtype=np.int64), np.zeros(self.NT - 1, dtype=np.int64)
         _check(self._L.pspin_batch_get_exchange_stats(self._h, s, a, p))
         return a, p
+
+
+class LPConfig(C.Structure):
+    """pspin_lp_config (include/pspin.h)."""
+    _fields_ = [("hidden", C.c_int), ("batch_cap", C.c_int), ("data_cap", C.c_longlong),
+                ("log_sigma0", C.c_double), ("log_sigma_min", C.c_double), ("log_sigma_max", C.c_double),
+                ("scale", C.c_double), ("init_gain", C.c_double), ("seed", C.c_uint64)]
+
+    @classmethod
+    def default(cls, **kw) -> "LPConfig":
+        c = cls()
+        _check(load_library().pspin_lp_default_config(C.byref(c)))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class LearnedMove:
+    """Learned global-move proposal attached to a dense PSpin state."""
+
+    def __init__(self, st: PSpin, cfg: LPConfig = None):
+        self._L = load_library()
+        self.st = st
+        self.cfg = cfg if cfg is not None else LPConfig.default()
+        self._h = C.c_void_p()
+        _check(self._L.pspin_lp_create(st._h, C.byref(self.cfg), C.byref(self._h)))
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(self._L.pspin_lp_dims(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        self.in_dim, self.in_pad, self.hidden, self.out_dim = a.value, b.value, c.value, d.value
+        self.n_params = int(self._L.pspin_lp_n_params(self._h))
+
+    def close(self):
+        if self._h:
+            self._L.pspin_lp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, dtype=np.float32)
+        _check(self._L.pspin_lp_get_params(self._h, p))
+        return p
+
+    def set_params(self, p) -> None:
+        _check(self._L.pspin_lp_set_params(self._h, np.ascontiguousarray(p, dtype=np.float32)))
+
+    def set_scale(self, scale: float) -> None:
+        _check(self._L.pspin_lp_set_scale(self._h, float(scale)))
+        self.cfg.scale = float(scale)
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros((x.shape[0], self.out_dim), dtype=np.float32)
+        _check(self._L.pspin_lp_forward(self._h, x, x.shape[0], out))
+        return out
+
+    def loss_grad(self, x, u, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        loss = C.c_double()
+        g = np.zeros(self.n_params, dtype=np.float32)
+        _check(self._L.pspin_lp_loss_grad(self._h, x, np.ascontiguousarray(u, dtype=np.float32),
+                                          np.ascontiguousarray(y, dtype=np.float32), x.shape[0], C.byref(loss), g))
+        return loss.value, g
+
+    def record(self) -> None:
+        _check(self._L.pspin_lp_record(self._h))
+
+    def collect(self, n_snapshots: int, every: int = 1, pt_freq: int = 1) -> None:
+        _check(self._L.pspin_lp_collect(self._h, int(n_snapshots), int(every), int(pt_freq)))
+
+    def data_size(self) -> int:
+        return int(self._L.pspin_lp_data_size(self._h))
+
+    def train(self, n_steps: int, batch: int = 1024, lag: int = 1, lr: float = 1e-3) -> np.ndarray:
+        hist = np.zeros(max(1, int(n_steps)))
+        _check(self._L.pspin_lp_train(self._h, int(n_steps), int(batch), int(lag), float(lr), _ptr(hist)))
+        return hist[:n_steps]
+
+    def move(self) -> None:
+        _check(self._L.pspin_lp_move(self._h))
+
+    def run(self, n_sweeps: int, pt_freq: int = 1, move_every: int = 8) -> None:
+        _check(self._L.pspin_lp_run(self._h, int(n_sweeps), int(pt_freq), int(move_every)))
+
+    def stats(self):
+        n = self.st.NT if self.st.NT else 1
+        a, p = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)
+        _check(self._L.pspin_lp_get_stats(self._h, a, p))
+        return a, p
+
+    def reset_stats(self) -> None:
+        _check(self._L.pspin_lp_reset_stats(self._h))
+
+    def last(self):
+        """(proposal [R][N] complex, dbg [R][8], out_fwd [R][4N], out_rev [R][4N])."""
+        R, N = self.st.nrep, self.st.N
+        prop = np.zeros((R, N, 2))
+        dbg = np.zeros((R, 8))
+        of = np.zeros((R, self.out_dim), dtype=np.float32)
+        orv = np.zeros((R, self.out_dim), dtype=np.float32)
+        _check(self._L.pspin_lp_get_last(self._h, _ptr(prop), _ptr(dbg), _ptr(of), _ptr(orv)))
+        return prop[..., 0] + 1j * prop[..., 1], dbg, of, orv
+
+    def move_counters(self) -> np.ndarray:
+        c = np.zeros(self.st.nrep, dtype=np.uint64)
+        _check(self._L.pspin_lp_get_move_counters(self._h, c))
+        return c

@@ -229,6 +229,75 @@ int pspin_set_kernel(pspin_state* st, int kernel, int block_size);
  * (g ~ hi + lo, ~16 mantissa bits) with FP32 accumulation. */
 int pspin_set_coupling_format(pspin_state* st, int format);
 
+/* ------------------------------------------------------------------------
+ * Learned global-move proposal.  It has no counterpart in the reference,
+ * which has only the pair-rotation move (include/spins.h:17-52).
+ *
+ * An MLP (2N+1 inputs: Re/Im of the amplitudes in a gauge-fixed frame plus
+ * beta; two SiLU hidden layers; 4N outputs) gives the mean and log-scale of
+ * a diagonal Gaussian over the 2N real coordinates.  A draw is projected
+ * radially onto sum|a|^2 = N and accepted by Metropolis-Hastings against
+ * pi ~ exp(-beta H) prod_k |a_k|^-1, the stationary law of the pair-rotation
+ * sweep, with the exact projected-Gaussian density and FP64 full energies.
+ * Dense (spherical) states only; N a multiple of 16.
+ * ---------------------------------------------------------------------- */
+typedef struct pspin_learned pspin_learned;
+typedef struct pspin_lp_config {
+    int hidden;           /* hidden width, multiple of 64 (512)                    */
+    int batch_cap;        /* largest training batch, multiple of 64 (4096)         */
+    long long data_cap;   /* snapshot capacity: data_cap x nrep configurations     */
+    double log_sigma0;    /* added to the log-scale output (log 0.05)              */
+    double log_sigma_min; /* log-scale clamp (-9, 1)                               */
+    double log_sigma_max;
+    double scale;         /* proposal scale factor on sigma (1)                    */
+    double init_gain;     /* scale of the output layer's initial weights (0.01)    */
+    uint64_t seed;        /* weight initialisation and batch sampling              */
+} pspin_lp_config;
+
+int pspin_lp_default_config(pspin_lp_config* cfg);
+int pspin_lp_create(pspin_state* st, const pspin_lp_config* cfg, pspin_learned** out);
+int pspin_lp_destroy(pspin_learned* lp);
+/* Flat FP32 parameters [W1 (hidden x in_pad) b1 W2 b2 W3 (4N x hidden) b3];
+ * in_pad = 2N+1 rounded up to 64, padding columns of W1 are zero. */
+long long pspin_lp_n_params(const pspin_learned* lp);
+int pspin_lp_dims(const pspin_learned* lp, int* in_dim, int* in_pad, int* hidden, int* out_dim);
+int pspin_lp_get_params(const pspin_learned* lp, float* out);
+int pspin_lp_set_params(pspin_learned* lp, const float* in); /* also resets Adam */
+int pspin_lp_set_scale(pspin_learned* lp, double scale);
+/* Network evaluation on host rows x [M][in_dim] (M <= batch_cap): FP32
+ * outputs [M][4N]. */
+int pspin_lp_forward(pspin_learned* lp, const float* x, int M, float* out);
+/* Gaussian negative log-likelihood of targets y [M][2N] given inputs x
+ * [M][in_dim] and gauge-fixed amplitudes u [M][2N] (mean = u + output):
+ * mean loss per row and the gradient of the mean loss [n_params]. */
+int pspin_lp_loss_grad(pspin_learned* lp, const float* x, const float* u, const float* y, int M, double* loss,
+                       float* grad);
+/* Training data: record every replica's amplitudes and beta (one snapshot),
+ * or run PT (pspin_pt_run) recording a snapshot every `every` sweeps. */
+int pspin_lp_record(pspin_learned* lp);
+int pspin_lp_collect(pspin_learned* lp, long long n_snapshots, int every, int pt_freq);
+long long pspin_lp_data_size(const pspin_learned* lp);
+/* n_steps Adam steps on random (snapshot t, replica r) -> (t + lag, r)
+ * pairs; loss_hist [n_steps] (may be NULL) receives the batch losses. */
+int pspin_lp_train(pspin_learned* lp, long long n_steps, int batch, int lag, double lr, double* loss_hist);
+/* One learned move for every replica (exact MH); energies are resynced to
+ * the FP64 full energy as a side effect. */
+int pspin_lp_move(pspin_learned* lp);
+/* PT with learned moves: n_sweeps sweeps, exchange every pt_freq sweeps,
+ * one learned move after every `move_every` sweeps (move_every a multiple
+ * of pt_freq). */
+int pspin_lp_run(pspin_learned* lp, long long n_sweeps, int pt_freq, int move_every);
+/* Accepted / proposed learned moves per temperature slot [NT] (one slot
+ * before pspin_pt_init); reset with pspin_lp_reset_stats. */
+int pspin_lp_get_stats(const pspin_learned* lp, long long* acc, long long* prop);
+int pspin_lp_reset_stats(pspin_learned* lp);
+/* Last move, per replica: proposal amplitudes [R][N][2] (may be NULL) and
+ * dbg [R][8] = log q(a'|a), log q(a|a'), E(a), E(a'), log MH ratio,
+ * accepted, log w(a), log w(a') with w = prod_k |a_k|^-1; net outputs at
+ * the current state and at the proposal [R][4N] (may be NULL). */
+int pspin_lp_get_last(const pspin_learned* lp, double* prop, double* dbg, float* out_fwd, float* out_rev);
+int pspin_lp_get_move_counters(const pspin_learned* lp, uint64_t* ctr);
+
 #ifdef __cplusplus
 }
 #endif

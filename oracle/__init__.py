@@ -30,6 +30,7 @@ _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 
 def build(force: bool = False) -> None:
@@ -86,6 +87,14 @@ def lib() -> C.CDLL:
             "or_max_threads": (C.c_int, []),
             "or_cpu_pt": (None, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _u8p, _dp, _u64p, C.c_int, C.c_long,
                                  C.c_int, _dp, _dp, _dp, _i64p, _i64p, _i64p, _i64p, C.POINTER(C.c_int)]),
+            "or_log_jn": (C.c_double, [C.c_int, C.c_double]),
+            "or_lp_log_density": (C.c_double, [C.c_int, _dp, _dp, _dp]),
+            "or_lp_gauge": (C.c_int, [C.c_int, _dp, _dp, _dp]),
+            "or_lp_draws": (None, [C.c_uint64, C.c_uint64, C.c_int, _dp, C.POINTER(C.c_double)]),
+            "or_lp_propose": (C.c_double, [C.c_int, _dp, _f32p, C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.c_uint64, C.c_uint64, _dp]),
+            "or_lp_accept": (C.c_int, [C.c_int, C.c_double, _dp, _dp, C.c_double, C.c_double, _f32p, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -294,4 +303,49 @@ def cpu_pt(N, NT, nr, spins, g2, h2_active, g4, mask, beta_sched, rseeds, n_swee
                     out["ex_acc"], out["ex_prop"], C.byref(n))
     out["n"] = n.value
     out["spins"] = sp
+    return out
+
+
+# ----------------------------------------------------------------------------
+# Learned global move (projected-Gaussian proposal, exact MH)
+# ----------------------------------------------------------------------------
+def log_jn(n: int, alpha: float) -> float:
+    """log of int_0^inf s^n exp(-s^2/2 + alpha s) ds."""
+    return lib().or_log_jn(int(n), float(alpha))
+
+
+def lp_log_density(y, mu, ls) -> float:
+    """log density of unit vector y under the radially projected N(mu, diag exp(2 ls))."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().or_lp_log_density(len(y), y, np.ascontiguousarray(mu, dtype=np.float64),
+                                   np.ascontiguousarray(ls, dtype=np.float64))
+
+
+def lp_gauge(N: int, a):
+    """(k*, c, u): gauge phase c = conj(a_k*)/|a_k*| and u = a c as [N][2]."""
+    c = np.zeros(2)
+    u = np.zeros(2 * N)
+    k = lib().or_lp_gauge(N, _spins(a), c, u)
+    return k, c, u
+
+
+def lp_draws(seed: int, move: int, N: int):
+    xi = np.zeros(2 * N)
+    u = C.c_double()
+    lib().or_lp_draws(seed, move, N, xi, C.byref(u))
+    return xi, u.value
+
+
+def lp_propose(N, a, O_f, ls0, ls_min, ls_max, log_scale, seed, move):
+    prop = np.zeros(2 * N)
+    lq = lib().or_lp_propose(N, _spins(a), np.ascontiguousarray(O_f, dtype=np.float32).reshape(-1), ls0, ls_min,
+                             ls_max, log_scale, seed, move, prop)
+    return prop, lq
+
+
+def lp_accept(N, beta, a, prop, E, E_new, O_r, ls0, ls_min, ls_max, log_scale, lq_f, seed, move):
+    out = np.zeros(8)
+    lib().or_lp_accept(N, beta, _spins(a), _spins(prop), E, E_new,
+                       np.ascontiguousarray(O_r, dtype=np.float32).reshape(-1), ls0, ls_min, ls_max, log_scale,
+                       lq_f, seed, move, out)
     return out

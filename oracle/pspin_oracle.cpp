@@ -24,6 +24,11 @@
 //   pt_exchange           glibc drand48 itself; hand-traced 3-temperature case
 //   partial_shuffle       permutation property + uniform-selection statistics
 //   philox4x32_10         Random123 known-answer vectors
+//   log_jn / lp density   closed form at alpha = 0, recurrence, quadrature,
+//                         sphere normalisation, uniform special case
+//   learned move (MH)     stationary law on the CPU: Dirichlet(1/2) intensities
+//                         at beta = 0 and agreement with the pair-rotation
+//                         chain at beta > 0
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -720,6 +725,172 @@ void or_cpu_pt(int N, int NT, int nr, double* spins, const double* g2, int h2_ac
             (*n_meas)++;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Learned global move (this build's own design; the reference has only the
+// pair rotation).  Restated here so the GPU move can be replayed and its
+// stationary law checked on the CPU.
+//
+//  gauge      u = a c, c = conj(a_k*)/|a_k*|, k* = argmax |a_k|^2 (first)
+//  proposal   z = mu + sigma xi in the frame, y = z/|z|, a' = sqrt(N) y conj(c)
+//             mu = u + O[0:2N], log sigma = clamp(O[2N:4N] + ls0) + log_scale
+//  density    projected Gaussian on the unit sphere (w.r.t. surface measure)
+//             p(y) = (2pi)^(-d/2) prod sigma^-1 e^(-C/2) A^(-d/2) J_{d-1}(B/sqrt A)
+//  MH         log r = -beta (E' - E) + log w(a') - log w(a) + log q(a|a') - log q(a'|a),
+//             w = prod_k |a_k|^-1 (the pair-rotation chain's stationary density)
+//
+// log J_n(alpha) = log int_0^inf s^n exp(-s^2/2 + alpha s) ds by the
+// trapezoid rule in t = log s (8 nodes per Laplace width in t) -- a
+// different variable and window from the device code's trapezoid in s.  Pinned against the closed form at alpha = 0, the
+// three-term recurrence J_n = alpha J_{n-1} + (n-1) J_{n-2} for alpha >= 0 and
+// adaptive quadrature for alpha < 0 (tests/test_oracle_learned.py).
+// ---------------------------------------------------------------------------
+double or_log_jn(int n, double alpha) {
+    // s = e^t: J_n = int exp(g(t)) dt, g(t) = (n+1) t - e^(2t)/2 + alpha e^t,
+    // smooth and decaying on both sides, so the trapezoid rule converges
+    // exponentially for every n >= 0.
+    const double m = (double)n + 1.0;
+    const double sq = std::sqrt(alpha * alpha + 4.0 * m);
+    const double so = alpha >= 0.0 ? 0.5 * (alpha + sq) : 2.0 * m / (sq - alpha);  // e^t at the maximum
+    const double to = std::log(so);
+    const double go = m * to - 0.5 * so * so + alpha * so;
+    const double wt = 1.0 / std::sqrt(so * (2.0 * so - alpha));
+    const double h = wt / 8.0;
+    const double left = std::max(40.0 * wt, 110.0 / m), right = 40.0 * wt;
+    const long long jl = (long long)std::ceil(left / h), jr = (long long)std::ceil(right / h);
+    double sum = 0.0;
+    for (long long j = -jl; j <= jr; j++) {
+        const double t = to + h * (double)j;
+        const double e = std::exp(t);
+        sum += std::exp(m * t - 0.5 * e * e + alpha * e - go);
+    }
+    return go + std::log(sum * h);
+}
+
+double or_lp_log_density(int d, const double* y, const double* mu, const double* ls) {
+    double A = 0.0, B = 0.0, C = 0.0, L = 0.0;
+    for (int i = 0; i < d; i++) {
+        const double iv = std::exp(-2.0 * ls[i]);
+        A += y[i] * y[i] * iv;
+        B += y[i] * mu[i] * iv;
+        C += mu[i] * mu[i] * iv;
+        L += ls[i];
+    }
+    const double hd = 0.5 * (double)d;
+    return -hd * std::log(2.0 * M_PI) - L - 0.5 * C - hd * std::log(A) + or_log_jn(d - 1, B / std::sqrt(A));
+}
+
+int or_lp_gauge(int N, const double* a, double* c, double* u) {
+    int bk = 0;
+    double best = -1.0;
+    for (int k = 0; k < N; k++) {
+        const double x = a[2 * k] * a[2 * k] + a[2 * k + 1] * a[2 * k + 1];
+        if (x > best) {
+            best = x;
+            bk = k;
+        }
+    }
+    const double r = std::sqrt(best);
+    c[0] = a[2 * bk] / r;
+    c[1] = -a[2 * bk + 1] / r;
+    if (u)
+        for (int k = 0; k < N; k++) {
+            u[2 * k] = a[2 * k] * c[0] - a[2 * k + 1] * c[1];
+            u[2 * k + 1] = a[2 * k] * c[1] + a[2 * k + 1] * c[0];
+        }
+    return bk;
+}
+
+// Normals xi[0..2N) and the accept uniform of move `mv` of a replica.
+void or_lp_draws(uint64_t seed, uint64_t mv, int N, double* xi, double* u_acc) {
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int b = 0; b < N; b++) {
+        uint32_t ctr[4] = {(uint32_t)mv, (uint32_t)(mv >> 32), (uint32_t)b, 32u}, x[4];
+        or_philox4x32_10(ctr, key, x);
+        const double u1 = or_u64_to_unit(x[0], x[1]), u2 = or_u64_to_unit(x[2], x[3]);
+        const double rad = std::sqrt(-2.0 * std::log(u1));
+        xi[2 * b] = rad * std::cos(2.0 * M_PI * u2);
+        xi[2 * b + 1] = rad * std::sin(2.0 * M_PI * u2);
+    }
+    if (u_acc) {
+        uint32_t ctr[4] = {(uint32_t)mv, (uint32_t)(mv >> 32), 0xFFFFFFFFu, 32u}, x[4];
+        or_philox4x32_10(ctr, key, x);
+        *u_acc = or_u64_to_unit(x[0], x[1]);
+    }
+}
+
+static void lp_mu_ls(int N, const double* u, const float* O, double ls0, double ls_min, double ls_max,
+                     double log_scale, double* mu, double* ls) {
+    const int d = 2 * N;
+    for (int i = 0; i < d; i++) {
+        mu[i] = u[i] + (double)O[i];
+        ls[i] = std::min(std::max((double)O[d + i] + ls0, ls_min), ls_max) + log_scale;
+    }
+}
+
+// Proposal of move `mv` from a [N][2] given the network output O_f [4N] at
+// a: writes prop [N][2], returns log q(a'|a).
+double or_lp_propose(int N, const double* a, const float* O_f, double ls0, double ls_min, double ls_max,
+                     double log_scale, uint64_t seed, uint64_t mv, double* prop) {
+    const int d = 2 * N;
+    std::vector<double> u(d), mu(d), ls(d), xi(d), z(d);
+    double c[2];
+    or_lp_gauge(N, a, c, u.data());
+    lp_mu_ls(N, u.data(), O_f, ls0, ls_min, ls_max, log_scale, mu.data(), ls.data());
+    or_lp_draws(seed, mv, N, xi.data(), nullptr);
+    double z2 = 0.0;
+    for (int i = 0; i < d; i++) {
+        z[i] = mu[i] + std::exp(ls[i]) * xi[i];
+        z2 += z[i] * z[i];
+    }
+    const double inv = 1.0 / std::sqrt(z2);
+    for (int i = 0; i < d; i++) z[i] *= inv;
+    const double lq = or_lp_log_density(d, z.data(), mu.data(), ls.data());
+    const double rn = std::sqrt((double)N);
+    for (int k = 0; k < N; k++) {
+        const double re = z[2 * k] * rn, im = z[2 * k + 1] * rn;
+        prop[2 * k] = re * c[0] + im * c[1];
+        prop[2 * k + 1] = im * c[0] - re * c[1];
+    }
+    return lq;
+}
+
+// MH decision of move `mv`: out[8] = log q(a'|a), log q(a|a'), E, E', log r,
+// accepted, log w(a), log w(a').  O_r is the network output at the proposal.
+int or_lp_accept(int N, double beta, const double* a, const double* prop, double E, double E_new,
+                 const float* O_r, double ls0, double ls_min, double ls_max, double log_scale, double lq_f,
+                 uint64_t seed, uint64_t mv, double* out) {
+    const int d = 2 * N;
+    std::vector<double> up(d), mu(d), ls(d), v(d);
+    double c[2];
+    or_lp_gauge(N, prop, c, up.data());
+    lp_mu_ls(N, up.data(), O_r, ls0, ls_min, ls_max, log_scale, mu.data(), ls.data());
+    double v2 = 0.0, lw = 0.0, lwp = 0.0;
+    for (int k = 0; k < N; k++) {
+        v[2 * k] = a[2 * k] * c[0] - a[2 * k + 1] * c[1];
+        v[2 * k + 1] = a[2 * k] * c[1] + a[2 * k + 1] * c[0];
+        v2 += v[2 * k] * v[2 * k] + v[2 * k + 1] * v[2 * k + 1];
+        lw -= 0.5 * std::log(a[2 * k] * a[2 * k] + a[2 * k + 1] * a[2 * k + 1]);
+        lwp -= 0.5 * std::log(prop[2 * k] * prop[2 * k] + prop[2 * k + 1] * prop[2 * k + 1]);
+    }
+    const double inv = 1.0 / std::sqrt(v2);
+    for (int i = 0; i < d; i++) v[i] *= inv;
+    const double lq_r = or_lp_log_density(d, v.data(), mu.data(), ls.data());
+    const double logr = -beta * (E_new - E) + (lwp - lw) + (lq_r - lq_f);
+    double u;
+    std::vector<double> xi(d);
+    or_lp_draws(seed, mv, N, xi.data(), &u);
+    const int acc = std::log(u) < logr;
+    out[0] = lq_f;
+    out[1] = lq_r;
+    out[2] = E;
+    out[3] = E_new;
+    out[4] = logr;
+    out[5] = acc;
+    out[6] = lw;
+    out[7] = lwp;
+    return acc;
 }
 
 int or_max_threads(void) {
