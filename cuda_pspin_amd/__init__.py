@@ -155,6 +155,7 @@ def load_library() -> C.CDLL:
         "pspin_lp_get_params": (C.c_int, [P, f32p]),
         "pspin_lp_set_params": (C.c_int, [P, f32p]),
         "pspin_lp_set_scale": (C.c_int, [P, C.c_double]),
+        "pspin_lp_set_mean_scale": (C.c_int, [P, C.c_double]),
         "pspin_lp_forward": (C.c_int, [P, f32p, C.c_int, f32p]),
         "pspin_lp_loss_grad": (C.c_int, [P, f32p, f32p, f32p, C.c_int, C.POINTER(C.c_double), f32p]),
         "pspin_lp_record": (C.c_int, [P]),
@@ -436,7 +437,8 @@ class LPConfig(C.Structure):
     """pspin_lp_config (include/pspin.h)."""
     _fields_ = [("hidden", C.c_int), ("batch_cap", C.c_int), ("data_cap", C.c_longlong),
                 ("log_sigma0", C.c_double), ("log_sigma_min", C.c_double), ("log_sigma_max", C.c_double),
-                ("scale", C.c_double), ("init_gain", C.c_double), ("seed", C.c_uint64)]
+                ("scale", C.c_double), ("init_gain", C.c_double), ("seed", C.c_uint64),
+                ("mean_scale", C.c_double)]
 
     @classmethod
     def default(cls, **kw) -> "LPConfig":
@@ -487,6 +489,10 @@ class LearnedMove:
     def set_scale(self, scale: float) -> None:
         _check(self._L.pspin_lp_set_scale(self._h, float(scale)))
         self.cfg.scale = float(scale)
+
+    def set_mean_scale(self, m: float) -> None:
+        _check(self._L.pspin_lp_set_mean_scale(self._h, float(m)))
+        self.cfg.mean_scale = float(m)
 
     def forward(self, x) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -161,6 +161,7 @@ ps::MoveArgs move_args(pspin_learned* lp) {
     a.N = lp->N; a.R = lp->R; a.in_pad = lp->in_pad; a.out_ld = lp->out_dim;
     a.ls0 = lp->cfg.log_sigma0; a.ls_min = lp->cfg.log_sigma_min; a.ls_max = lp->cfg.log_sigma_max;
     a.log_scale = std::log(lp->cfg.scale);
+    a.mean_scale = lp->cfg.mean_scale;
     a.spins = st->d_spins; a.spins_w = st->d_spins; a.betas = st->d_betas; a.E = st->d_E;
     a.seeds = st->d_seeds; a.mctr = lp->d_mctr;
     a.X0 = lp->fw.X0; a.X0p = lp->rv.X0;
@@ -291,6 +292,7 @@ int pspin_lp_default_config(pspin_lp_config* cfg) {
     cfg->scale = 1.0;
     cfg->init_gain = 0.01;
     cfg->seed = 12345;
+    cfg->mean_scale = 1.0;
     return PSPIN_OK;
 }
 
@@ -401,6 +403,12 @@ int pspin_lp_set_params(pspin_learned* lp, const float* in) {
 int pspin_lp_set_scale(pspin_learned* lp, double scale) {
     LP_REQUIRE(lp && scale > 0.0, PSPIN_ERR_ARG, "scale must be > 0");
     lp->cfg.scale = scale;
+    return PSPIN_OK;
+}
+
+int pspin_lp_set_mean_scale(pspin_learned* lp, double mean_scale) {
+    LP_REQUIRE(lp && mean_scale >= 0.0, PSPIN_ERR_ARG, "mean scale must be >= 0");
+    lp->cfg.mean_scale = mean_scale;
     return PSPIN_OK;
 }
 

@@ -17,6 +17,9 @@
 //
 // Projected Gaussian.  In the frame, z ~ N(mu, diag sigma^2) over the d = 2N
 // real coordinates and y = z / |z| on the unit sphere; a' = sqrt(N) y conj(c).
+// mu = u + m O[0:2N] and log sigma = clamp(O[2N:4N] + ls0) + log s, with the
+// proposal scale s and the mean-offset scale m (m = s^2 keeps a learned
+// Langevin-like drift consistent with the shrunken step).
 // The density of y w.r.t. the surface measure of the unit sphere is
 //   p(y) = (2 pi)^(-d/2) prod sigma_i^-1 exp(-C/2) A^(-d/2) J_{d-1}(B / sqrt A),
 //   A = sum y_i^2/sigma_i^2, B = sum y_i mu_i/sigma_i^2, C = sum mu_i^2/sigma_i^2,
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(64) void k_lp_propose(MoveArgs a) {
         ls[t] = 0.0;
         z[t] = 0.0;
         if (d < dim) {
-            mu[t] = U[d] + (double)O[d];
+            mu[t] = U[d] + a.mean_scale * (double)O[d];
             ls[t] = clampd((double)O[dim + d] + a.ls0, a.ls_min, a.ls_max) + a.log_scale;
             const uint32_t b = (uint32_t)(d >> 1);
             const U4 x = philox4x32_10(U4{(uint32_t)mv, (uint32_t)(mv >> 32), b, kCtrLearned}, k0, k1);
@@ -346,7 +349,7 @@ __global__ __launch_bounds__(64) void k_lp_accept(MoveArgs a) {
             const double re = S[2 * k], im = S[2 * k + 1];
             v[t] = (d & 1) ? re * ci + im * cr : re * cr - im * ci;
             v2 += v[t] * v[t];
-            mu[t] = Up[d] + (double)O[d];
+            mu[t] = Up[d] + a.mean_scale * (double)O[d];
             ls[t] = clampd((double)O[dim + d] + a.ls0, a.ls_min, a.ls_max) + a.log_scale;
             if (!(d & 1)) {
                 lw -= 0.5 * log(re * re + im * im);

@@ -48,7 +48,7 @@ hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, long long n, hipS
 // Kernel arguments of one learned move over R replicas (one wave each).
 struct MoveArgs {
     int N, R, in_pad, out_ld;
-    double ls0, ls_min, ls_max, log_scale;
+    double ls0, ls_min, ls_max, log_scale, mean_scale;
     const double* spins;  // [R][N][2] current amplitudes
     double* spins_w;      // same buffer, written on acceptance
     const double* betas;

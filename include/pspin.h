@@ -252,6 +252,7 @@ typedef struct pspin_lp_config {
     double scale;         /* proposal scale factor on sigma (1)                    */
     double init_gain;     /* scale of the output layer's initial weights (0.01)    */
     uint64_t seed;        /* weight initialisation and batch sampling              */
+    double mean_scale;    /* factor on the mean offset: mean = u + mean_scale*out (1) */
 } pspin_lp_config;
 
 int pspin_lp_default_config(pspin_lp_config* cfg);
@@ -264,6 +265,7 @@ int pspin_lp_dims(const pspin_learned* lp, int* in_dim, int* in_pad, int* hidden
 int pspin_lp_get_params(const pspin_learned* lp, float* out);
 int pspin_lp_set_params(pspin_learned* lp, const float* in); /* also resets Adam */
 int pspin_lp_set_scale(pspin_learned* lp, double scale);
+int pspin_lp_set_mean_scale(pspin_learned* lp, double mean_scale);
 /* Network evaluation on host rows x [M][in_dim] (M <= batch_cap): FP32
  * outputs [M][4N]. */
 int pspin_lp_forward(pspin_learned* lp, const float* x, int M, float* out);

@@ -92,9 +92,10 @@ def lib() -> C.CDLL:
             "or_lp_gauge": (C.c_int, [C.c_int, _dp, _dp, _dp]),
             "or_lp_draws": (None, [C.c_uint64, C.c_uint64, C.c_int, _dp, C.POINTER(C.c_double)]),
             "or_lp_propose": (C.c_double, [C.c_int, _dp, _f32p, C.c_double, C.c_double, C.c_double, C.c_double,
-                                           C.c_uint64, C.c_uint64, _dp]),
+                                           C.c_double, C.c_uint64, C.c_uint64, _dp]),
             "or_lp_accept": (C.c_int, [C.c_int, C.c_double, _dp, _dp, C.c_double, C.c_double, _f32p, C.c_double,
-                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64, _dp]),
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                       C.c_uint64, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -336,16 +337,16 @@ def lp_draws(seed: int, move: int, N: int):
     return xi, u.value
 
 
-def lp_propose(N, a, O_f, ls0, ls_min, ls_max, log_scale, seed, move):
+def lp_propose(N, a, O_f, ls0, ls_min, ls_max, log_scale, seed, move, mean_scale=1.0):
     prop = np.zeros(2 * N)
     lq = lib().or_lp_propose(N, _spins(a), np.ascontiguousarray(O_f, dtype=np.float32).reshape(-1), ls0, ls_min,
-                             ls_max, log_scale, seed, move, prop)
+                             ls_max, log_scale, mean_scale, seed, move, prop)
     return prop, lq
 
 
-def lp_accept(N, beta, a, prop, E, E_new, O_r, ls0, ls_min, ls_max, log_scale, lq_f, seed, move):
+def lp_accept(N, beta, a, prop, E, E_new, O_r, ls0, ls_min, ls_max, log_scale, lq_f, seed, move, mean_scale=1.0):
     out = np.zeros(8)
     lib().or_lp_accept(N, beta, _spins(a), _spins(prop), E, E_new,
                        np.ascontiguousarray(O_r, dtype=np.float32).reshape(-1), ls0, ls_min, ls_max, log_scale,
-                       lq_f, seed, move, out)
+                       mean_scale, lq_f, seed, move, out)
     return out

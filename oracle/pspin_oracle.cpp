@@ -734,7 +734,7 @@ void or_cpu_pt(int N, int NT, int nr, double* spins, const double* g2, int h2_ac
 //
 //  gauge      u = a c, c = conj(a_k*)/|a_k*|, k* = argmax |a_k|^2 (first)
 //  proposal   z = mu + sigma xi in the frame, y = z/|z|, a' = sqrt(N) y conj(c)
-//             mu = u + O[0:2N], log sigma = clamp(O[2N:4N] + ls0) + log_scale
+//             mu = u + m O[0:2N], log sigma = clamp(O[2N:4N] + ls0) + log_scale
 //  density    projected Gaussian on the unit sphere (w.r.t. surface measure)
 //             p(y) = (2pi)^(-d/2) prod sigma^-1 e^(-C/2) A^(-d/2) J_{d-1}(B/sqrt A)
 //  MH         log r = -beta (E' - E) + log w(a') - log w(a) + log q(a|a') - log q(a'|a),
@@ -821,10 +821,10 @@ void or_lp_draws(uint64_t seed, uint64_t mv, int N, double* xi, double* u_acc) {
 }
 
 static void lp_mu_ls(int N, const double* u, const float* O, double ls0, double ls_min, double ls_max,
-                     double log_scale, double* mu, double* ls) {
+                     double log_scale, double mean_scale, double* mu, double* ls) {
     const int d = 2 * N;
     for (int i = 0; i < d; i++) {
-        mu[i] = u[i] + (double)O[i];
+        mu[i] = u[i] + mean_scale * (double)O[i];
         ls[i] = std::min(std::max((double)O[d + i] + ls0, ls_min), ls_max) + log_scale;
     }
 }
@@ -832,12 +832,12 @@ static void lp_mu_ls(int N, const double* u, const float* O, double ls0, double 
 // Proposal of move `mv` from a [N][2] given the network output O_f [4N] at
 // a: writes prop [N][2], returns log q(a'|a).
 double or_lp_propose(int N, const double* a, const float* O_f, double ls0, double ls_min, double ls_max,
-                     double log_scale, uint64_t seed, uint64_t mv, double* prop) {
+                     double log_scale, double mean_scale, uint64_t seed, uint64_t mv, double* prop) {
     const int d = 2 * N;
     std::vector<double> u(d), mu(d), ls(d), xi(d), z(d);
     double c[2];
     or_lp_gauge(N, a, c, u.data());
-    lp_mu_ls(N, u.data(), O_f, ls0, ls_min, ls_max, log_scale, mu.data(), ls.data());
+    lp_mu_ls(N, u.data(), O_f, ls0, ls_min, ls_max, log_scale, mean_scale, mu.data(), ls.data());
     or_lp_draws(seed, mv, N, xi.data(), nullptr);
     double z2 = 0.0;
     for (int i = 0; i < d; i++) {
@@ -859,13 +859,14 @@ double or_lp_propose(int N, const double* a, const float* O_f, double ls0, doubl
 // MH decision of move `mv`: out[8] = log q(a'|a), log q(a|a'), E, E', log r,
 // accepted, log w(a), log w(a').  O_r is the network output at the proposal.
 int or_lp_accept(int N, double beta, const double* a, const double* prop, double E, double E_new,
-                 const float* O_r, double ls0, double ls_min, double ls_max, double log_scale, double lq_f,
+                 const float* O_r, double ls0, double ls_min, double ls_max, double log_scale,
+                 double mean_scale, double lq_f,
                  uint64_t seed, uint64_t mv, double* out) {
     const int d = 2 * N;
     std::vector<double> up(d), mu(d), ls(d), v(d);
     double c[2];
     or_lp_gauge(N, prop, c, up.data());
-    lp_mu_ls(N, up.data(), O_r, ls0, ls_min, ls_max, log_scale, mu.data(), ls.data());
+    lp_mu_ls(N, up.data(), O_r, ls0, ls_min, ls_max, log_scale, mean_scale, mu.data(), ls.data());
     double v2 = 0.0, lw = 0.0, lwp = 0.0;
     for (int k = 0; k < N; k++) {
         v[2 * k] = a[2 * k] * c[0] - a[2 * k + 1] * c[1];
