@@ -345,6 +345,7 @@ constexpr int kGraphSweeps = 32;  // sweeps per captured graph (rounded to whole
 void drop_graph(pspin_state* st) {
     if (st->graph) (void)hipGraphExecDestroy(st->graph);
     st->graph = nullptr;
+    st->graph_gen++;
     st->graph_pt_freq = st->graph_rounds = 0;
 }
 
@@ -383,6 +384,49 @@ int maybe_resync(pspin_state* st) {
 }
 
 }  // namespace
+
+namespace ps {
+// PT sweeps and exchanges queued on the library stream, no final sync (the
+// body of pspin_pt_run; also used between learned moves).
+int pt_advance(pspin_state* st, int n_sweeps, int pt_freq) {
+    PS_REQUIRE(st && n_sweeps >= 0 && pt_freq >= 1, PSPIN_ERR_ARG, "bad argument");
+    PS_REQUIRE(st->pt_NT >= 2, PSPIN_ERR_STATE, "pspin_pt_init has not been called");
+    int done = 0;
+    // Whole rounds (pt_freq sweeps + one exchange) go through a captured
+    // graph of kGraphRounds rounds when nothing per-launch is being recorded.
+    const bool can_graph = !st->prof && !st->d_log;
+    const int rounds = std::max(1, kGraphSweeps / pt_freq);
+    while (done < n_sweeps) {
+        const int left = n_sweeps - done;
+        if (can_graph && st->pt_sweeps % pt_freq == 0 && left >= rounds * pt_freq) {
+            int rc = ensure_graph(st, pt_freq, rounds);
+            if (rc) return rc;
+            PS_TRY_HIP(hipGraphLaunch(st->graph, st->stream));
+            const int nsw = rounds * pt_freq;
+            st->sweep_counter += (uint64_t)nsw;
+            st->pt_sweeps += nsw;
+            done += nsw;
+            st->since_resync += nsw;
+            if ((rc = maybe_resync(st))) return rc;
+            continue;
+        }
+        // Sweeps until the next exchange point ((s+1) % pt_freq == 0).
+        long long to_next = pt_freq - (st->pt_sweeps % pt_freq);
+        int chunk = (int)std::min<long long>(to_next, left);
+        int rc = sweep_launch(st, chunk);
+        if (rc) return rc;
+        st->pt_sweeps += chunk;
+        done += chunk;
+        st->since_resync += chunk;
+        if ((rc = maybe_resync(st))) return rc;
+        if (st->pt_sweeps % pt_freq == 0) {
+            rc = exchange_launch(st);
+            if (rc) return rc;
+        }
+    }
+    return PSPIN_OK;
+}
+}  // namespace ps
 
 extern "C" {
 
@@ -815,41 +859,8 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
 }
 
 int pspin_pt_run(pspin_state* st, int n_sweeps, int pt_freq) {
-    PS_REQUIRE(st && n_sweeps >= 0 && pt_freq >= 1, PSPIN_ERR_ARG, "bad argument");
-    PS_REQUIRE(st->pt_NT >= 2, PSPIN_ERR_STATE, "pspin_pt_init has not been called");
-    int done = 0;
-    // Whole rounds (pt_freq sweeps + one exchange) go through a captured
-    // graph of kGraphRounds rounds when nothing per-launch is being recorded.
-    const bool can_graph = !st->prof && !st->d_log;
-    const int rounds = std::max(1, kGraphSweeps / pt_freq);
-    while (done < n_sweeps) {
-        const int left = n_sweeps - done;
-        if (can_graph && st->pt_sweeps % pt_freq == 0 && left >= rounds * pt_freq) {
-            int rc = ensure_graph(st, pt_freq, rounds);
-            if (rc) return rc;
-            PS_TRY_HIP(hipGraphLaunch(st->graph, st->stream));
-            const int nsw = rounds * pt_freq;
-            st->sweep_counter += (uint64_t)nsw;
-            st->pt_sweeps += nsw;
-            done += nsw;
-            st->since_resync += nsw;
-            if ((rc = maybe_resync(st))) return rc;
-            continue;
-        }
-        // Sweeps until the next exchange point ((s+1) % pt_freq == 0).
-        long long to_next = pt_freq - (st->pt_sweeps % pt_freq);
-        int chunk = (int)std::min<long long>(to_next, left);
-        int rc = sweep_launch(st, chunk);
-        if (rc) return rc;
-        st->pt_sweeps += chunk;
-        done += chunk;
-        st->since_resync += chunk;
-        if ((rc = maybe_resync(st))) return rc;
-        if (st->pt_sweeps % pt_freq == 0) {
-            rc = exchange_launch(st);
-            if (rc) return rc;
-        }
-    }
+    const int rc = ps::pt_advance(st, n_sweeps, pt_freq);
+    if (rc) return rc;
     PS_TRY_HIP(hipStreamSynchronize(st->stream));
     return PSPIN_OK;
 }

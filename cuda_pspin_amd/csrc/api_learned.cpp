@@ -80,6 +80,7 @@ void free_act(pspin_mlp_act& a) {
 void free_lp(pspin_learned* lp) {
     free_act(lp->tr); free_act(lp->fw); free_act(lp->rv);
     dfree(lp->d_p); dfree(lp->d_m); dfree(lp->d_v); dfree(lp->d_g); dfree(lp->d_pb);
+    dfree(lp->d_gpart); dfree(lp->d_bpart);
     dfree(lp->d_data); dfree(lp->d_dbeta);
     dfree(lp->d_u); dfree(lp->d_up); dfree(lp->d_gc); dfree(lp->d_gcp); dfree(lp->d_prop); dfree(lp->d_dbg);
     dfree(lp->d_part); dfree(lp->d_mctr); dfree(lp->d_lacc); dfree(lp->d_lprop); dfree(lp->d_terms);
@@ -89,6 +90,8 @@ void free_lp(pspin_learned* lp) {
 
 hipError_t sync(const pspin_learned* lp) { return hipStreamSynchronize(lp->st->stream); }
 
+constexpr int kMaxSplits = 8;
+
 // Forward pass over M (multiple of 64) rows of act.X0.
 hipError_t forward(pspin_learned* lp, pspin_mlp_act& a, int M) {
     const hipStream_t s = lp->st->stream;
@@ -97,62 +100,65 @@ hipError_t forward(pspin_learned* lp, pspin_mlp_act& a, int M) {
     g.B = lp->d_pb + lp->off_w1; g.b_n = lp->in_pad; g.b_k = 1;
     g.M = M; g.N = lp->hid; g.K = lp->in_pad;
     g.bias = lp->d_p + lp->off_b1; g.Cf = a.Z1; g.ldc = lp->hid; g.Cb = a.H1; g.ldcb = lp->hid;
-    hipError_t e = ps::launch_gemm(g, ps::kEpiBiasSilu, s);
+    hipError_t e = ps::launch_gemm(g, ps::kEpiBiasSilu, 1, s);
     if (e) return e;
     g.A = a.H1; g.a_m = lp->hid;
     g.B = lp->d_pb + lp->off_w2; g.b_n = lp->hid;
     g.K = lp->hid;
     g.bias = lp->d_p + lp->off_b2; g.Cf = a.Z2; g.Cb = a.H2;
-    if ((e = ps::launch_gemm(g, ps::kEpiBiasSilu, s))) return e;
+    if ((e = ps::launch_gemm(g, ps::kEpiBiasSilu, 1, s))) return e;
     g.A = a.H2;
     g.B = lp->d_pb + lp->off_w3;
     g.N = lp->out_dim;
     g.bias = lp->d_p + lp->off_b3; g.Cf = a.O; g.ldc = lp->out_dim; g.Cb = nullptr;
-    return ps::launch_gemm(g, ps::kEpiBiasF32, s);
+    return ps::launch_gemm(g, ps::kEpiBiasF32, 1, s);
+}
+
+// K splits of a weight-gradient product over M batch rows: the largest
+// power of two <= kMaxSplits with M a multiple of 64 x splits.
+int grad_splits(int M) {
+    int sp = kMaxSplits;
+    while (sp > 1 && M % (64 * sp)) sp >>= 1;
+    return sp;
+}
+
+// dW [rows][cols] = dZ^T X and db [rows] = column sums of dZ, dZ [M][rows]
+// and X [M][cols] BF16, into the flat gradient at (off_w, off_b).
+hipError_t weight_grad(pspin_learned* lp, const uint16_t* dZ, int rows, const uint16_t* X, int cols, int M,
+                       long long off_w, long long off_b) {
+    const hipStream_t s = lp->st->stream;
+    const int sp = grad_splits(M);
+    ps::GemmArgs g{};
+    g.A = dZ; g.a_m = 1; g.a_k = rows;
+    g.B = X; g.b_n = 1; g.b_k = cols;
+    g.M = rows; g.N = cols; g.K = M;
+    g.Cf = sp > 1 ? lp->d_gpart : lp->d_g + off_w; g.ldc = cols;
+    g.rowsum = sp > 1 ? lp->d_bpart : lp->d_g + off_b;
+    hipError_t e = ps::launch_gemm(g, ps::kEpiF32, sp, s);
+    if (e || sp == 1) return e;
+    if ((e = ps::launch_sum_slices(lp->d_gpart, sp, (long long)rows * cols, lp->d_g + off_w, s))) return e;
+    return ps::launch_sum_slices(lp->d_bpart, sp, rows, lp->d_g + off_b, s);
+}
+
+// dX = (dZ W) * silu'(Z) into BF16 out, dZ [M][n_out], W [n_out][n_in].
+hipError_t data_grad(pspin_learned* lp, const uint16_t* dZ, int n_out, long long off_w, int n_in, const float* Z,
+                     uint16_t* out, int M) {
+    ps::GemmArgs g{};
+    g.A = dZ; g.a_m = n_out; g.a_k = 1;
+    g.B = lp->d_pb + off_w; g.b_n = 1; g.b_k = n_in;
+    g.M = M; g.N = n_in; g.K = n_out;
+    g.Z = Z; g.ldz = n_in; g.Cb = out; g.ldcb = n_in;
+    return ps::launch_gemm(g, ps::kEpiDSilu, 1, lp->st->stream);
 }
 
 // Backward pass from act.dO (BF16) into lp->d_g.
 hipError_t backward(pspin_learned* lp, pspin_mlp_act& a, int M) {
-    const hipStream_t s = lp->st->stream;
     hipError_t e;
-    ps::GemmArgs g{};
-    // dW3 = dO^T H2   [out_dim][hid]
-    g.A = a.dO; g.a_m = 1; g.a_k = lp->out_dim;
-    g.B = a.H2; g.b_n = 1; g.b_k = lp->hid;
-    g.M = lp->out_dim; g.N = lp->hid; g.K = M;
-    g.Cf = lp->d_g + lp->off_w3; g.ldc = lp->hid;
-    if ((e = ps::launch_gemm(g, ps::kEpiF32, s))) return e;
-    if ((e = ps::launch_colsum_bf16(a.dO, M, lp->out_dim, lp->out_dim, lp->d_g + lp->off_b3, s))) return e;
-    // dZ2 = (dO W3) * silu'(Z2)   [M][hid]
-    g = ps::GemmArgs{};
-    g.A = a.dO; g.a_m = lp->out_dim; g.a_k = 1;
-    g.B = lp->d_pb + lp->off_w3; g.b_n = 1; g.b_k = lp->hid;
-    g.M = M; g.N = lp->hid; g.K = lp->out_dim;
-    g.Z = a.Z2; g.ldz = lp->hid; g.Cb = a.dZ2; g.ldcb = lp->hid;
-    if ((e = ps::launch_gemm(g, ps::kEpiDSilu, s))) return e;
-    // dW2 = dZ2^T H1
-    g = ps::GemmArgs{};
-    g.A = a.dZ2; g.a_m = 1; g.a_k = lp->hid;
-    g.B = a.H1; g.b_n = 1; g.b_k = lp->hid;
-    g.M = lp->hid; g.N = lp->hid; g.K = M;
-    g.Cf = lp->d_g + lp->off_w2; g.ldc = lp->hid;
-    if ((e = ps::launch_gemm(g, ps::kEpiF32, s))) return e;
-    if ((e = ps::launch_colsum_bf16(a.dZ2, M, lp->hid, lp->hid, lp->d_g + lp->off_b2, s))) return e;
-    // dZ1 = (dZ2 W2) * silu'(Z1)
-    g = ps::GemmArgs{};
-    g.A = a.dZ2; g.a_m = lp->hid; g.a_k = 1;
-    g.B = lp->d_pb + lp->off_w2; g.b_n = 1; g.b_k = lp->hid;
-    g.M = M; g.N = lp->hid; g.K = lp->hid;
-    g.Z = a.Z1; g.ldz = lp->hid; g.Cb = a.dZ1; g.ldcb = lp->hid;
-    if ((e = ps::launch_gemm(g, ps::kEpiDSilu, s))) return e;
-    // dW1 = dZ1^T X0   [hid][in_pad]
-    g = ps::GemmArgs{};
-    g.A = a.dZ1; g.a_m = 1; g.a_k = lp->hid;
-    g.B = a.X0; g.b_n = 1; g.b_k = lp->in_pad;
-    g.M = lp->hid; g.N = lp->in_pad; g.K = M;
-    g.Cf = lp->d_g + lp->off_w1; g.ldc = lp->in_pad;
-    if ((e = ps::launch_gemm(g, ps::kEpiF32, s))) return e;
-    return ps::launch_colsum_bf16(a.dZ1, M, lp->hid, lp->hid, lp->d_g + lp->off_b1, s);
+    if ((e = weight_grad(lp, a.dO, lp->out_dim, a.H2, lp->hid, M, lp->off_w3, lp->off_b3))) return e;
+    if ((e = data_grad(lp, a.dO, lp->out_dim, lp->off_w3, lp->hid, a.Z2, a.dZ2, M))) return e;
+    if ((e = weight_grad(lp, a.dZ2, lp->hid, a.H1, lp->hid, M, lp->off_w2, lp->off_b2))) return e;
+    if ((e = data_grad(lp, a.dZ2, lp->hid, lp->off_w2, lp->hid, a.Z1, a.dZ1, M))) return e;
+    return weight_grad(lp, a.dZ1, lp->hid, a.X0, lp->in_pad, M, lp->off_w1, lp->off_b1);
 }
 
 ps::MoveArgs move_args(pspin_learned* lp) {
@@ -174,9 +180,8 @@ ps::MoveArgs move_args(pspin_learned* lp) {
     return a;
 }
 
-hipError_t move_launch(pspin_learned* lp) {
+hipError_t move_launch(pspin_learned* lp, const ps::MoveArgs& a) {
     const hipStream_t s = lp->st->stream;
-    const ps::MoveArgs a = move_args(lp);
     hipError_t e;
     if ((e = ps::launch_lp_prep(a, s))) return e;
     if ((e = forward(lp, lp->fw, lp->Rpad))) return e;
@@ -184,6 +189,39 @@ hipError_t move_launch(pspin_learned* lp) {
     if ((e = forward(lp, lp->rv, lp->Rpad))) return e;
     if ((e = ps::launch_lp_energy(a, lp->d_terms, lp->n_terms, lp->d_part, s))) return e;
     return ps::launch_lp_accept(a, s);
+}
+
+// Capture (every / pt_freq rounds of pt_freq sweeps + one exchange, then one
+// learned move) into lp->graph unless the current capture matches.  All
+// kernels read their counters and PT state from device memory, so replays
+// continue the chains.
+int ensure_graph(pspin_learned* lp, int pt_freq, int every) {
+    const ps::MoveArgs a = move_args(lp);
+    if (lp->graph && lp->graph_pt_freq == pt_freq && lp->graph_every == every &&
+        lp->graph_gen == lp->st->graph_gen && std::memcmp(&a, &lp->graph_args, sizeof(a)) == 0)
+        return PSPIN_OK;
+    if (lp->graph) (void)hipGraphExecDestroy(lp->graph);
+    lp->graph = nullptr;
+    pspin_state* st = lp->st;
+    hipGraph_t g = nullptr;
+    LP_TRY_HIP(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; every > 0 && k < every / pt_freq && e == hipSuccess; k++) {
+        e = ps::launch_sweep(st, pt_freq);
+        if (e == hipSuccess) e = ps::launch_exchange(st);
+    }
+    if (e == hipSuccess) e = move_launch(lp, a);
+    hipError_t e2 = hipStreamEndCapture(st->stream, &g);
+    LP_TRY_HIP(e);
+    LP_TRY_HIP(e2);
+    hipError_t e3 = hipGraphInstantiate(&lp->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    LP_TRY_HIP(e3);
+    lp->graph_pt_freq = pt_freq;
+    lp->graph_every = every;
+    lp->graph_args = a;
+    lp->graph_gen = st->graph_gen;
+    return PSPIN_OK;
 }
 
 // FP64 Hamiltonian terms of the state: H4 quartets (colex walk of the dense
@@ -226,9 +264,10 @@ int build_terms(pspin_learned* lp) {
     LP_TRY_HIP(dmalloc(&lp->d_terms, terms.size()));
     if (!terms.empty())
         LP_TRY_HIP(hipMemcpy(lp->d_terms, terms.data(), terms.size() * sizeof(uint4), hipMemcpyHostToDevice));
-    const int nb = (2 * lp->R + 7) / 8;
-    const long long by_terms = std::max(1LL, (lp->n_terms + 1023) / 1024);
-    lp->n_split = (int)std::max(1LL, std::min<long long>(by_terms, (1024 + nb - 1) / nb));
+    // each split is one workgroup's worth of terms (every thread one chunk)
+    const long long per = ps::lp_energy_terms_per_split();
+    lp->n_split = (int)std::max(1LL, (lp->n_terms + per - 1) / per);
+    LP_TRY_HIP(ps::lp_energy_prepare());
     LP_TRY_HIP(dmalloc(&lp->d_part, (size_t)lp->n_split * 2 * lp->R));
     return PSPIN_OK;
 }
@@ -300,7 +339,8 @@ int pspin_lp_create(pspin_state* st, const pspin_lp_config* cfg, pspin_learned**
     LP_REQUIRE(st && cfg && out, PSPIN_ERR_ARG, "null argument");
     *out = nullptr;
     LP_REQUIRE(st->variant == PSPIN_DENSE, PSPIN_ERR_ARG, "the learned move needs the spherical (dense) variant");
-    LP_REQUIRE(st->N % 16 == 0 && st->N <= ps::kMaxN, PSPIN_ERR_ARG, "the learned move needs N a multiple of 16");
+    LP_REQUIRE(st->N % 16 == 0 && st->N <= ps::kLearnedMaxN, PSPIN_ERR_ARG,
+               "the learned move needs N a multiple of 16, at most 128");
     LP_REQUIRE(cfg->hidden >= 64 && cfg->hidden % 64 == 0, PSPIN_ERR_ARG, "hidden must be a positive multiple of 64");
     LP_REQUIRE(cfg->batch_cap >= 64 && cfg->batch_cap % 64 == 0, PSPIN_ERR_ARG,
                "batch_cap must be a positive multiple of 64");
@@ -338,6 +378,10 @@ int pspin_lp_create(pspin_state* st, const pspin_lp_config* cfg, pspin_learned**
     if (!e) e = dmalloc(&lp->d_v, np);
     if (!e) e = dmalloc(&lp->d_g, np);
     if (!e) e = dmalloc(&lp->d_pb, np);
+    const long long wmax = std::max({(long long)lp->out_dim * lp->hid, (long long)lp->hid * lp->hid,
+                                     (long long)lp->hid * lp->in_pad});
+    if (!e) e = dmalloc(&lp->d_gpart, (size_t)kMaxSplits * wmax);
+    if (!e) e = dmalloc(&lp->d_bpart, (size_t)kMaxSplits * std::max(lp->out_dim, lp->hid));
     if (!e) e = alloc_act(lp->tr, cfg->batch_cap, lp, true);
     if (!e) e = alloc_act(lp->fw, lp->Rpad, lp, false);
     if (!e) e = alloc_act(lp->rv, lp->Rpad, lp, false);
@@ -515,8 +559,11 @@ int pspin_lp_train(pspin_learned* lp, long long n_steps, int batch, int lag, dou
 
 int pspin_lp_move(pspin_learned* lp) {
     LP_REQUIRE(lp, PSPIN_ERR_ARG, "null handle");
-    LP_TRY_HIP(move_launch(lp));
+    int rc = ensure_graph(lp, 1, 0);
+    if (rc) return rc;
+    LP_TRY_HIP(hipGraphLaunch(lp->graph, lp->st->stream));
     LP_TRY_HIP(sync(lp));
+    lp->st->since_resync = 0;  // the move leaves every running energy at its FP64 value
     return PSPIN_OK;
 }
 
@@ -525,13 +572,26 @@ int pspin_lp_run(pspin_learned* lp, long long n_sweeps, int pt_freq, int move_ev
     LP_REQUIRE(move_every % pt_freq == 0, PSPIN_ERR_ARG, "move_every must be a multiple of pt_freq");
     pspin_state* st = lp->st;
     LP_REQUIRE(st->pt_NT >= 2, PSPIN_ERR_STATE, "pspin_pt_init has not been called");
+    const bool can_graph = !st->prof && !st->d_log;
     long long done = 0;
     while (done < n_sweeps) {
         const int chunk = (int)std::min<long long>(move_every, n_sweeps - done);
-        int rc = pspin_pt_run(st, chunk, pt_freq);
-        if (rc) return rc;
+        if (chunk == move_every && can_graph && st->pt_sweeps % pt_freq == 0) {
+            int rc = ensure_graph(lp, pt_freq, move_every);
+            if (rc) return rc;
+            LP_TRY_HIP(hipGraphLaunch(lp->graph, st->stream));
+            st->sweep_counter += (uint64_t)chunk;
+            st->pt_sweeps += chunk;
+            st->since_resync = 0;
+        } else {
+            int rc = ps::pt_advance(st, chunk, pt_freq);
+            if (rc) return rc;
+            if (chunk == move_every) {
+                LP_TRY_HIP(move_launch(lp, move_args(lp)));
+                st->since_resync = 0;
+            }
+        }
         done += chunk;
-        if (chunk == move_every) LP_TRY_HIP(move_launch(lp));
     }
     LP_TRY_HIP(sync(lp));
     return PSPIN_OK;

@@ -255,70 +255,111 @@ __global__ __launch_bounds__(64) void k_lp_propose(MoveArgs a) {
     }
 }
 
-// Full FP64 energies of 2R configurations (current states, then proposals),
-// RB configurations per workgroup sharing each term read; the term list is
-// split over gridDim.y and partial sums are combined in fixed order later.
-constexpr int kEnRB = 8;
-constexpr int kEnBS = 256;
-
 struct c64d {
     double x, y;
 };
 __device__ __forceinline__ c64d cmul(c64d p, c64d q) { return {p.x * q.x - p.y * q.y, p.x * q.y + p.y * q.x}; }
 __device__ __forceinline__ c64d ccj(c64d p) { return {p.x, -p.y}; }
 
-__global__ __launch_bounds__(kEnBS) void k_lp_energy(const double* __restrict__ cur, const double* __restrict__ prop,
-                                                     int R, int N, const uint4* __restrict__ terms, long long nt,
-                                                     double* __restrict__ part) {
-    __shared__ c64d sa[kEnRB][kMaxN];
-    __shared__ double red[kEnBS / 64][kEnRB];
-    const int c0 = blockIdx.x * kEnRB;
-    for (int e = threadIdx.x; e < kEnRB * N; e += kEnBS) {
-        const int c = c0 + e / N, k = e % N;
+// Full FP64 energies of 2R configurations (current states, then proposals).
+// A workgroup takes kEnCfg configurations (amplitudes in LDS as
+// [site][config]) and a range of the term list; each thread walks kEnChunk
+// consecutive terms for all kEnCfg configurations.  The list is in colex
+// order (l, k, j outer to inner), so a thread keeps the products of a_k, a_l
+// and of a_j with them across its terms and a quartet term costs one
+// amplitude read and two FMAs per configuration:
+//   ch0  a_i a_j a_k* a_l* = a_i [a_j (a_k a_l)*]
+//   ch1  a_i a_j* a_k* a_l = a_i [a_j* (a_k* a_l)]
+//   ch2  a_i a_j* a_k a_l* = a_i [a_j* (a_k a_l*)]
+// Pair terms (ch 3) read both amplitudes.  The term list is split over
+// gridDim.y; the accept kernel combines the partial sums in fixed order.
+constexpr int kEnCfg = 4;
+constexpr int kEnChunk = 16;
+constexpr int kEnThreads = 256;
+
+__global__ __launch_bounds__(kEnThreads) void k_lp_energy(const double* __restrict__ cur,
+                                                          const double* __restrict__ prop, int R, int N,
+                                                          const uint4* __restrict__ terms, long long nt,
+                                                          double* __restrict__ part) {
+    __shared__ c64d sl[kLearnedMaxN][kEnCfg];
+    __shared__ double red[kEnThreads / 64][kEnCfg];
+    const int c0 = blockIdx.x * kEnCfg;
+    for (int e = threadIdx.x; e < kEnCfg * N; e += kEnThreads) {
+        const int q = e / N, k = e - q * N, c = c0 + q;
         c64d v = {0.0, 0.0};
         if (c < 2 * R) {
             const double* src = c < R ? cur + (size_t)2 * N * c : prop + (size_t)2 * N * (c - R);
             v = {src[2 * k], src[2 * k + 1]};
         }
-        sa[e / N][k] = v;
+        sl[k][q] = v;
     }
     __syncthreads();
-    const long long chunk = (nt + gridDim.y - 1) / gridDim.y;
-    const long long t0 = chunk * blockIdx.y, t1 = min(nt, t0 + chunk);
-    double acc[kEnRB];
+    const long long per_block = (long long)kEnThreads * kEnChunk;
+    const long long t0 = ((long long)blockIdx.y * kEnThreads + threadIdx.x) * kEnChunk;
+    const long long t1 = min(nt, t0 + kEnChunk);
+    (void)per_block;
+    double en[kEnCfg];
+    c64d qkl[kEnCfg], rkl[kEnCfg], skl[kEnCfg];
+    double p0x[kEnCfg], p0y[kEnCfg], p1x[kEnCfg], p1y[kEnCfg], p2x[kEnCfg], p2y[kEnCfg];
 #pragma unroll
-    for (int q = 0; q < kEnRB; q++) acc[q] = 0.0;
-    for (long long t = t0 + threadIdx.x; t < t1; t += kEnBS) {
+    for (int q = 0; q < kEnCfg; q++) en[q] = 0.0;
+    uint32_t ckl = 0xFFFFFFFFu, cj = 0xFFFFFFFFu;
+    for (long long t = t0; t < t1; t++) {
         const uint4 w = terms[t];
         const double g = __hiloint2double((int)w.w, (int)w.z);
-        const int i = w.x & 255, j = (w.x >> 8) & 255, k = (w.x >> 16) & 255, l = w.x >> 24;
-        const int ch = (int)w.y;
+        const uint32_t i = w.x & 255u, j = (w.x >> 8) & 255u;
+        if (w.y == 3u) {
 #pragma unroll
-        for (int q = 0; q < kEnRB; q++) {
-            double re;
-            if (ch == 3) {
-                re = cmul(sa[q][i], ccj(sa[q][j])).x;
-            } else {
-                const c64d s1 = ch == 0 ? sa[q][j] : ccj(sa[q][j]);
-                const c64d s2 = ch == 2 ? sa[q][k] : ccj(sa[q][k]);
-                const c64d s3 = ch == 1 ? sa[q][l] : ccj(sa[q][l]);
-                re = cmul(cmul(sa[q][i], s1), cmul(s2, s3)).x;
+            for (int q = 0; q < kEnCfg; q++) {
+                const c64d ai = sl[i][q], aj = sl[j][q];
+                en[q] -= g * (ai.x * aj.x + ai.y * aj.y);
             }
-            acc[q] -= g * re;
+            continue;
+        }
+        const uint32_t kl = w.x >> 16;
+        if (kl != ckl) {
+#pragma unroll
+            for (int q = 0; q < kEnCfg; q++) {
+                const c64d ak = sl[kl & 255u][q], al = sl[kl >> 8][q];
+                qkl[q] = ccj(cmul(ak, al));
+                rkl[q] = cmul(ccj(ak), al);
+                skl[q] = cmul(ak, ccj(al));
+            }
+            ckl = kl;
+            cj = 0xFFFFFFFFu;
+        }
+        if (j != cj) {
+#pragma unroll
+            for (int q = 0; q < kEnCfg; q++) {
+                const c64d aj = sl[j][q];
+                const c64d u0 = cmul(aj, qkl[q]), u1 = cmul(ccj(aj), rkl[q]), u2 = cmul(ccj(aj), skl[q]);
+                p0x[q] = u0.x; p0y[q] = u0.y;
+                p1x[q] = u1.x; p1y[q] = u1.y;
+                p2x[q] = u2.x; p2y[q] = u2.y;
+            }
+            cj = j;
+        }
+        const bool c0sel = w.y == 0u, c1sel = w.y == 1u;
+#pragma unroll
+        for (int q = 0; q < kEnCfg; q++) {
+            const double px = c0sel ? p0x[q] : (c1sel ? p1x[q] : p2x[q]);
+            const double py = c0sel ? p0y[q] : (c1sel ? p1y[q] : p2y[q]);
+            const c64d ai = sl[i][q];
+            en[q] -= g * (ai.x * px - ai.y * py);
         }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < kEnRB; q++) {
-        const double v = wsum(acc[q]);
+    for (int q = 0; q < kEnCfg; q++) {
+        const double v = wsum(en[q]);
         if (lane == 0) red[wv][q] = v;
     }
     __syncthreads();
-    if (threadIdx.x < kEnRB) {
+    if (threadIdx.x < kEnCfg) {
         const int c = c0 + threadIdx.x;
         if (c < 2 * R) {
             double v = 0.0;
-            for (int x = 0; x < kEnBS / 64; x++) v += red[x][threadIdx.x];
+            for (int x = 0; x < kEnThreads / 64; x++) v += red[x][threadIdx.x];
             part[(size_t)blockIdx.y * 2 * R + c] = v;
         }
     }
@@ -328,10 +369,12 @@ __global__ __launch_bounds__(64) void k_lp_accept(MoveArgs a) {
     const int r = blockIdx.x, lane = threadIdx.x;
     const int N = a.N, dim = 2 * N, per = (dim + 63) / 64;
     double ec = 0.0, en = 0.0;
-    for (int s = 0; s < a.n_split; s++) {
+    for (int s = lane; s < a.n_split; s += 64) {
         ec += a.part[(size_t)s * 2 * a.R + r];
         en += a.part[(size_t)s * 2 * a.R + a.R + r];
     }
+    ec = wsum(ec);
+    en = wsum(en);
     // reverse density: current state in the proposal's frame, under the
     // network output at the proposal
     const double cr = a.gcp[2 * r], ci = a.gcp[2 * r + 1];
@@ -401,6 +444,11 @@ __global__ __launch_bounds__(64) void k_lp_accept(MoveArgs a) {
     }
 }
 
+hipError_t lp_energy_prepare() { return hipSuccess; }
+
+int lp_energy_blocks(int R) { return (2 * R + kEnCfg - 1) / kEnCfg; }
+long long lp_energy_terms_per_split() { return (long long)kEnThreads * kEnChunk; }
+
 hipError_t launch_lp_prep(const MoveArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_lp_prep, dim3(a.R), dim3(64), 0, s, a);
     return hipGetLastError();
@@ -410,9 +458,10 @@ hipError_t launch_lp_propose(const MoveArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t launch_lp_energy(const MoveArgs& a, const uint4* terms, long long nt, double* part, hipStream_t s) {
-    dim3 grid((2 * a.R + kEnRB - 1) / kEnRB, a.n_split);
-    hipLaunchKernelGGL(k_lp_energy, grid, dim3(kEnBS), 0, s, a.spins, (const double*)a.prop, a.R, a.N, terms, nt,
-                       part);
+    if (a.N > kLearnedMaxN) return hipErrorInvalidValue;
+    dim3 grid(lp_energy_blocks(a.R), a.n_split);
+    hipLaunchKernelGGL(k_lp_energy, grid, dim3(kEnThreads), 0, s, a.spins, (const double*)a.prop, a.R, a.N, terms,
+                       nt, part);
     return hipGetLastError();
 }
 hipError_t launch_lp_accept(const MoveArgs& a, hipStream_t s) {

@@ -6,12 +6,13 @@
 //   data gradient  dX = dZ W        A = dZ [M][Nout] (k-contiguous)   B = W        (n-contiguous)
 //   weight grad.   dW = dZ^T X      A = dZ           (m-contiguous)   B = X        (n-contiguous)
 // A 256-thread workgroup computes a 64x64 tile with four waves, each a
-// 32x32 block accumulated by v_mfma_f32_32x32x16_bf16 over K steps of 32;
+// 32x32 block accumulated by v_mfma_f32_32x32x16_bf16 over K steps of 64;
 // operand tiles are staged in LDS as [row][k] with a padded pitch so each
 // lane's 8-element fragment is one 16-byte LDS read.  Epilogues fuse the
 // bias, the SiLU activation (storing the FP32 pre-activation for the
 // backward pass and the BF16 activation for the next layer) and the SiLU
-// derivative of the data gradient.
+// derivative of the data gradient; weight-gradient products (K = batch
+// rows) split K over gridDim.z and also produce the bias gradient.
 #include "pspin_learned.h"
 
 namespace ps {
@@ -33,51 +34,99 @@ __device__ __forceinline__ float dsilu(float z) {
 }
 
 constexpr int kTile = 64;
-constexpr int kTK = 32;
-constexpr int kPitch = kTK + 8;  // 80-byte rows: 16-byte aligned, staggered banks
+constexpr int kTK = 64;
+constexpr int kPitch = kTK + 8;  // 144-byte rows: 16-byte aligned, staggered banks
 
-// Stage a 64 x 32 tile (rows r0.., k0..) of P into L[row][k].
-// KC: P is contiguous along k (s = row stride); else contiguous along rows
-// (s = k stride).  Each thread moves one 16-byte vector.
+// Global -> registers: a 64 x 64 tile (rows r0.., k0..) of P, 16 elements
+// per thread.  KC: P contiguous along k (s = row stride); else contiguous
+// along rows (s = k stride).
 template <bool KC>
-__device__ __forceinline__ void stage(const uint16_t* __restrict__ P, long long s, int r0, int k0,
-                                      uint16_t (*L)[kPitch]) {
+__device__ __forceinline__ void fetch(const uint16_t* __restrict__ P, long long s, int r0, int k0, uint4& v0,
+                                      uint4& v1) {
+    const int t = threadIdx.x;
+    const uint16_t* q = KC ? P + (long long)(r0 + (t >> 2)) * s + (k0 + (t & 3) * 16)
+                           : P + (long long)(k0 + (t >> 2)) * s + (r0 + (t & 3) * 16);
+    v0 = *(const uint4*)q;
+    v1 = *(const uint4*)(q + 8);
+}
+
+__device__ __forceinline__ uint16_t half_of(const uint4& v, int i) {
+    const uint32_t w = i < 2 ? v.x : (i < 4 ? v.y : (i < 6 ? v.z : v.w));
+    return (uint16_t)((i & 1) ? (w >> 16) : (w & 0xFFFFu));
+}
+
+// Registers -> LDS tile L[row][k].
+template <bool KC>
+__device__ __forceinline__ void put(const uint4& v0, const uint4& v1, uint16_t (*L)[kPitch]) {
     const int t = threadIdx.x;
     if (KC) {
-        const int r = t >> 2, c = (t & 3) * 8;
-        *(uint4*)&L[r][c] = *(const uint4*)(P + (long long)(r0 + r) * s + (k0 + c));
+        const int r = t >> 2, c = (t & 3) * 16;
+        *(uint4*)&L[r][c] = v0;
+        *(uint4*)&L[r][c + 8] = v1;
     } else {
-        const int k = t >> 3, r = (t & 7) * 8;
-        const uint4 v = *(const uint4*)(P + (long long)(k0 + k) * s + (r0 + r));
-        const uint16_t* e = (const uint16_t*)&v;
+        const int k = t >> 2, r = (t & 3) * 16;
 #pragma unroll
-        for (int i = 0; i < 8; i++) L[r + i][k] = e[i];
+        for (int i = 0; i < 8; i++) {
+            L[r + i][k] = half_of(v0, i);
+            L[r + 8 + i][k] = half_of(v1, i);
+        }
     }
 }
 
+// Block tile 64 x 64, four waves of 32 x 32, K steps of 64 with the next
+// step's operands fetched into registers while the current step's MFMAs run
+// (double-buffered LDS, one barrier per step).  gridDim.z > 1 splits K:
+// slice z covers [z K/Z, (z+1) K/Z) and writes FP32 partials (kEpiF32 only).
+// When rowsum is set, workgroups of the first column block also sum the
+// rows of A over their K slice (the bias gradient of a weight-gradient
+// product, whose A is the layer's output gradient transposed).
 template <bool AKC, bool BKC, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs a) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[kTile][kPitch];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[kTile][kPitch];
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][kTile][kPitch];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kTile][kPitch];
     const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     const int fr = lane & 31, fk = (lane >> 5) * 8;
     const long long sa = AKC ? a.a_m : a.a_k;
     const long long sb = BKC ? a.b_n : a.b_k;
+    const int kslice = a.K / (int)gridDim.z;
+    const int kb = blockIdx.z * kslice, ke = kb + kslice;
+    const bool rows = a.rowsum && blockIdx.x == 0 && threadIdx.x < kTile;
+    float rs = 0.0f;
     f32x16 acc = {};
-    for (int k0 = 0; k0 < a.K; k0 += kTK) {
-        stage<AKC>(a.A, sa, m0, k0, As);
-        stage<BKC>(a.B, sb, n0, k0, Bs);
-        __syncthreads();
+    uint4 ra0, ra1, rb0, rb1;
+    fetch<AKC>(a.A, sa, m0, kb, ra0, ra1);
+    fetch<BKC>(a.B, sb, n0, kb, rb0, rb1);
+    put<AKC>(ra0, ra1, As[0]);
+    put<BKC>(rb0, rb1, Bs[0]);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = kb; k0 < ke; k0 += kTK) {
+        const bool more = k0 + kTK < ke;
+        if (more) {
+            fetch<AKC>(a.A, sa, m0, k0 + kTK, ra0, ra1);
+            fetch<BKC>(a.B, sb, n0, k0 + kTK, rb0, rb1);
+        }
 #pragma unroll
         for (int kk = 0; kk < kTK; kk += 16) {
-            const bf16x8 fa = *(const bf16x8*)&As[wm + fr][kk + fk];
-            const bf16x8 fb = *(const bf16x8*)&Bs[wn + fr][kk + fk];
+            const bf16x8 fa = *(const bf16x8*)&As[cur][wm + fr][kk + fk];
+            const bf16x8 fb = *(const bf16x8*)&Bs[cur][wn + fr][kk + fk];
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc, 0, 0, 0);
         }
+        if (rows) {
+#pragma unroll 8
+            for (int kk = 0; kk < kTK; kk++) rs += __uint_as_float((uint32_t)As[cur][threadIdx.x][kk] << 16);
+        }
+        if (more) {
+            put<AKC>(ra0, ra1, As[cur ^ 1]);
+            put<BKC>(rb0, rb1, Bs[cur ^ 1]);
+        }
         __syncthreads();
+        cur ^= 1;
     }
+    if (rows) a.rowsum[(long long)blockIdx.z * a.M + m0 + threadIdx.x] = rs;
+    float* Cf = a.Cf + (long long)blockIdx.z * a.M * a.ldc;
     // accumulator element r of lane l: row 8(r/4) + 4(l/32) + r%4, column l%32
     const int n = n0 + wn + fr;
 #pragma unroll
@@ -85,7 +134,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs a) {
         const int m = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const float v = acc[r];
         if (EPI == kEpiF32) {
-            a.Cf[(long long)m * a.ldc + n] = v;
+            Cf[(long long)m * a.ldc + n] = v;
         } else if (EPI == kEpiBiasF32) {
             a.Cf[(long long)m * a.ldc + n] = v + a.bias[n];
         } else if (EPI == kEpiBiasSilu) {
@@ -100,8 +149,8 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs a) {
 }
 
 template <bool AKC, bool BKC>
-static hipError_t launch_epi(const GemmArgs& g, int epi, hipStream_t s) {
-    dim3 grid(g.N / kTile, g.M / kTile), block(256);
+static hipError_t launch_epi(const GemmArgs& g, int epi, int splits, hipStream_t s) {
+    dim3 grid(g.N / kTile, g.M / kTile, splits), block(256);
     switch (epi) {
     case kEpiF32: hipLaunchKernelGGL((k_gemm<AKC, BKC, kEpiF32>), grid, block, 0, s, g); break;
     case kEpiBiasF32: hipLaunchKernelGGL((k_gemm<AKC, BKC, kEpiBiasF32>), grid, block, 0, s, g); break;
@@ -112,33 +161,31 @@ static hipError_t launch_epi(const GemmArgs& g, int epi, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_gemm(const GemmArgs& g, int epi, hipStream_t s) {
+hipError_t launch_gemm(const GemmArgs& g, int epi, int splits, hipStream_t s) {
     const bool akc = g.a_k == 1, bkc = g.b_k == 1;
     if ((akc ? g.a_m : g.a_k) % 8 || (bkc ? g.b_n : g.b_k) % 8) return hipErrorInvalidValue;
     if (!akc && g.a_m != 1) return hipErrorInvalidValue;
     if (!bkc && g.b_n != 1) return hipErrorInvalidValue;
-    if (g.M % kTile || g.N % kTile || g.K % kTK || g.M <= 0 || g.N <= 0 || g.K <= 0) return hipErrorInvalidValue;
-    if (akc && bkc) return launch_epi<true, true>(g, epi, s);
-    if (akc && !bkc) return launch_epi<true, false>(g, epi, s);
-    if (!akc && !bkc) return launch_epi<false, false>(g, epi, s);
-    return launch_epi<false, true>(g, epi, s);
+    if (splits < 1 || (splits > 1 && epi != kEpiF32)) return hipErrorInvalidValue;
+    if (g.M % kTile || g.N % kTile || g.M <= 0 || g.N <= 0 || g.K <= 0 || g.K % (kTK * splits))
+        return hipErrorInvalidValue;
+    if (akc && bkc) return launch_epi<true, true>(g, epi, splits, s);
+    if (akc && !bkc) return launch_epi<true, false>(g, epi, splits, s);
+    if (!akc && !bkc) return launch_epi<false, false>(g, epi, splits, s);
+    return launch_epi<false, true>(g, epi, splits, s);
 }
 
-// Column sums of a BF16 [M][N] matrix (bias gradients), fixed order.
-__global__ __launch_bounds__(256) void k_colsum_bf16(const uint16_t* __restrict__ X, int M, long long ld,
-                                                     float* __restrict__ out) {
-    __shared__ float red[4][64];
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+// out[i] = sum_z part[z * n + i], z in order (split-K reduction).
+__global__ void k_sum_slices(const float* __restrict__ part, int nz, long long n, float* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
     float s = 0.0f;
-    for (int r = g; r < M; r += 4) s += __uint_as_float((uint32_t)X[(long long)r * ld + c] << 16);
-    red[g][threadIdx.x & 63] = s;
-    __syncthreads();
-    if (g == 0) out[c] = (red[0][c & 63] + red[1][c & 63]) + (red[2][c & 63] + red[3][c & 63]);
+    for (int z = 0; z < nz; z++) s += part[(long long)z * n + i];
+    out[i] = s;
 }
 
-hipError_t launch_colsum_bf16(const uint16_t* X, int M, int N, long long ld, float* out, hipStream_t s) {
-    if (N % 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_colsum_bf16, dim3(N / 64), dim3(256), 0, s, X, M, ld, out);
+hipError_t launch_sum_slices(const float* part, int nz, long long n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nz, n, out);
     return hipGetLastError();
 }
 

@@ -210,6 +210,7 @@ struct pspin_state {
     // captured sweep+exchange rounds
     hipGraphExec_t graph = nullptr;
     int graph_pt_freq = 0, graph_rounds = 0;
+    uint64_t graph_gen = 0;  // bumped whenever captured sweep graphs become stale
     double* d_e2 = nullptr;
     double* d_e4 = nullptr;
 
@@ -261,4 +262,7 @@ hipError_t launch_add(const double* a, const double* b, double* out, int n, hipS
 hipError_t launch_sweep(pspin_state* st, int n_sweeps);
 
 hipError_t launch_exchange(pspin_state* st);
+
+// pspin_pt_run without the final stream synchronisation (api.cpp).
+int pt_advance(pspin_state* st, int n_sweeps, int pt_freq);
 }  // namespace ps

@@ -23,7 +23,7 @@ enum GemmEpi { kEpiF32 = 0, kEpiBiasF32 = 1, kEpiBiasSilu = 2, kEpiDSilu = 3 };
 
 // C(m, n) = sum_k A(m, k) B(n, k), BF16 operands, FP32 accumulation.
 // A(m, k) = A[m * a_m + k * a_k]; exactly one of a_m, a_k is 1 (likewise B).
-// M, N multiples of 64; K multiple of 32; strides multiples of 8.
+// M, N multiples of 64; K a multiple of 64 x splits; strides multiples of 8.
 struct GemmArgs {
     const uint16_t* A;
     long long a_m, a_k;
@@ -37,10 +37,13 @@ struct GemmArgs {
     long long ldcb;
     const float* Z;     // pre-activation read by kEpiDSilu
     long long ldz;
+    float* rowsum;      // [splits][M] row sums of A over each K slice, or null
 };
 
-hipError_t launch_gemm(const GemmArgs& g, int epi, hipStream_t s);
-hipError_t launch_colsum_bf16(const uint16_t* X, int M, int N, long long ld, float* out, hipStream_t s);
+// splits > 1 (kEpiF32 only): Cf receives `splits` partial products, slice z
+// at Cf + z * M * ldc, to be summed by launch_sum_slices.
+hipError_t launch_gemm(const GemmArgs& g, int epi, int splits, hipStream_t s);
+hipError_t launch_sum_slices(const float* part, int nz, long long n, float* out, hipStream_t s);
 hipError_t launch_adam(float* p, float* m, float* v, const float* g, uint16_t* pb, long long n, float lr,
                        float b1, float b2, float eps, long long t, hipStream_t s);
 hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, long long n, hipStream_t s);
@@ -76,6 +79,10 @@ struct MoveArgs {
 hipError_t launch_lp_prep(const MoveArgs& a, hipStream_t s);
 hipError_t launch_lp_propose(const MoveArgs& a, hipStream_t s);
 hipError_t launch_lp_energy(const MoveArgs& a, const uint4* terms, long long nt, double* part, hipStream_t s);
+hipError_t lp_energy_prepare();  // once per process and device, before the first launch
+int lp_energy_blocks(int R);     // workgroups per term split
+long long lp_energy_terms_per_split();
+constexpr int kLearnedMaxN = 128;
 hipError_t launch_lp_accept(const MoveArgs& a, hipStream_t s);
 hipError_t launch_lp_record(const double* spins, const double* betas, int R, int dim, float* out, float* obeta,
                             hipStream_t s);
@@ -122,6 +129,8 @@ struct pspin_learned {
     float* d_g = nullptr;
     uint16_t* d_pb = nullptr;
     long long adam_t = 0;
+    float* d_gpart = nullptr;  // split-K partials of weight gradients [kSplits][max layer size]
+    float* d_bpart = nullptr;  // split-K partials of bias gradients [kSplits][max width]
 
     pspin_mlp_act tr;  // training batch
     pspin_mlp_act fw;  // move: forward pass at the current state
@@ -151,5 +160,9 @@ struct pspin_learned {
     uint4* d_terms = nullptr;
     long long n_terms = 0;
 
-    hipGraphExec_t graph = nullptr;  // one captured move
+    // captured (every/pt_freq PT rounds + one move); every == 0: the move alone
+    hipGraphExec_t graph = nullptr;
+    int graph_pt_freq = -1, graph_every = -1;
+    uint64_t graph_gen = 0;
+    ps::MoveArgs graph_args{};
 };

@@ -239,7 +239,7 @@ int pspin_set_coupling_format(pspin_state* st, int format);
  * radially onto sum|a|^2 = N and accepted by Metropolis-Hastings against
  * pi ~ exp(-beta H) prod_k |a_k|^-1, the stationary law of the pair-rotation
  * sweep, with the exact projected-Gaussian density and FP64 full energies.
- * Dense (spherical) states only; N a multiple of 16.
+ * Dense (spherical) states only; N a multiple of 16, at most 128.
  * ---------------------------------------------------------------------- */
 typedef struct pspin_learned pspin_learned;
 typedef struct pspin_lp_config {
