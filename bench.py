@@ -72,6 +72,115 @@ def ess_of_series(series):
     return ess, taus
 
 
+def ess_blocks(series):
+    """series [rounds, ladders] of an equilibrated block (no burn-in dropped):
+    ESS summed over ladders and the mean tau_int."""
+    ess, taus = 0.0, []
+    for r in range(series.shape[1]):
+        t = tau_int_sokal(series[:, r])
+        taus.append(t)
+        if np.isfinite(t):
+            ess += series.shape[0] / t
+    return ess, float(np.nanmean(taus))
+
+
+def tau_e_crossing(series):
+    """Reference-style tau_E: first lag where the normalised autocorrelation
+    falls below 1/e (src/analysis_pt.cpp:2694-2760), averaged over ladders."""
+    out = []
+    for r in range(series.shape[1]):
+        x = series[:, r] - series[:, r].mean()
+        n = len(x)
+        f = np.fft.rfft(x, 2 * n)
+        acf = np.fft.irfft(f * np.conj(f))[:n]
+        if acf[0] <= 0:
+            continue
+        rho = acf / acf[0]
+        below = np.nonzero(rho < math.exp(-1.0))[0]
+        if len(below):
+            out.append(float(below[0]))
+    return float(np.mean(out)) if out else None
+
+
+def learned_vs_plain(args):
+    """Energy ESS/s at the coldest slot with and without the learned global
+    move, on the N=64 uniform-FMC (gamma 0.05) system, 32 geometric
+    temperatures in [0.2,1.6] x 8 ladders, exchange every sweep.  The
+    network (2N+1 -> 512 -> 512 -> 4N) is trained on the GPU on 8192
+    snapshots of all 256 replicas (2^16 configurations per temperature) from
+    the plain run; the learned run mixes one move per `every` sweeps.  Plain
+    and learned production blocks alternate on the same equilibrated state, so
+    each rate has a block-to-block error."""
+    import cuda_pspin_amd as P
+
+    N, NT, nr = 64, 32, 8
+    R = NT * nr
+    T = np.array([args.Tmax * (args.Tmin / args.Tmax) ** (t / (NT - 1)) for t in range(NT)])
+    st = P.PSpin(P.SimConfig.default(N=N, nrep=R, seed=args.seed, fmc_mode=2, gamma=0.05, dev=0), P.DENSE)
+    st.pt_init(NT, nr, 1.0 / T, args.seed + 5000)
+    st.pt_run(args.lp_burn, 1)
+    lcfg = P.LPConfig.default(hidden=512, batch_cap=4096, data_cap=args.lp_snapshots, seed=args.seed)
+    lp = P.LearnedMove(st, lcfg)
+    t0 = time.perf_counter()
+    lp.collect(args.lp_snapshots, 1, 1)
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = lp.train(args.lp_train_steps, batch=4096, lag=1, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    lp.set_scale(args.lp_scale)
+    lp.set_mean_scale(args.lp_scale ** 2)
+    S = args.lp_block
+    rec = {"plain": [], "learned": []}
+    series_all = {"plain": [], "learned": []}
+    lp.reset_stats()
+    for b in range(args.lp_blocks):
+        for mode in ("plain", "learned"):
+            st.pt_enable_series(S)
+            t0 = time.perf_counter()
+            if mode == "plain":
+                st.pt_run(S, 1)
+            else:
+                lp.run(S, 1, args.lp_every)
+            w = time.perf_counter() - t0
+            ser = st.pt_series()
+            ess, tau = ess_blocks(ser)
+            rec[mode].append((w, ess, tau))
+            series_all[mode].append(ser)
+    acc, prop = lp.stats()
+    lp.close()
+    st.close()
+    out = {}
+    for mode in ("plain", "learned"):
+        w = np.array([x[0] for x in rec[mode]])
+        e = np.array([x[1] for x in rec[mode]])
+        rate = e / w
+        out[mode] = {"pair_updates_per_s": float(S * R * (N // 2) / w.mean()), "ms_per_sweep": float(w.mean() / S * 1e3),
+                     "energy_ess_per_s_tmin": float(rate.mean()),
+                     "energy_ess_per_s_tmin_stderr": float(rate.std(ddof=1) / math.sqrt(len(rate))) if len(rate) > 1 else None,
+                     "tau_int_sweeps": float(np.mean([x[2] for x in rec[mode]])),
+                     "tau_e_crossing_sweeps": tau_e_crossing(np.concatenate(series_all[mode], axis=0))}
+    prod_learned = float(np.sum([x[0] for x in rec["learned"]]))
+    ess_learned = float(np.sum([x[1] for x in rec["learned"]]))
+    out["learned"]["energy_ess_per_s_tmin_incl_training"] = ess_learned / (prod_learned + t_collect + t_train)
+    out["learned"]["move_acceptance_by_T"] = {f"{T[t]:.3f}": float(acc[t] / max(prop[t], 1)) for t in (0, NT // 2, NT - 1)}
+    flops_step = 2 * 4096 * ((2 * N + 1 + 63) // 64 * 64 * 512 + 512 * 512 + 512 * 4 * N) * 3
+    out["training"] = {"snapshots": args.lp_snapshots, "configs_per_T": args.lp_snapshots * nr,
+                       "collect_s": t_collect, "train_s": t_train, "steps": args.lp_train_steps, "batch": 4096,
+                       "loss_first": float(np.mean(hist[:20])), "loss_last": float(np.mean(hist[-20:])),
+                       "step_ms": t_train / args.lp_train_steps * 1e3,
+                       "gemm_tflops": flops_step / (t_train / args.lp_train_steps) / 1e12,
+                       "mfma_frac_bf16_dense": flops_step / (t_train / args.lp_train_steps) / 2.5e15}
+    r = np.array([x[1] / x[0] for x in rec["learned"]]) / np.array([x[1] / x[0] for x in rec["plain"]])
+    out["ess_ratio_learned_over_plain"] = float(r.mean())
+    out["ess_ratio_stderr"] = float(r.std(ddof=1) / math.sqrt(len(r))) if len(r) > 1 else None
+    out["config"] = {"workload": "dense spherical PT N=64 fmc uniform gamma=0.05, NT=32 x nrep=8, T in "
+                                 f"[{args.Tmin},{args.Tmax}] geometric, exchange every sweep",
+                     "learned": f"one move per {args.lp_every} sweeps, scale {args.lp_scale}, mean scale "
+                                f"{args.lp_scale ** 2:.3g}", "blocks": args.lp_blocks, "sweeps_per_block": S,
+                     "burn_in_sweeps": args.lp_burn}
+    return out
+
+
 def cpu_baseline(st, N, nrep, betas, seed, n_sweeps):
     import oracle as O
 
@@ -199,8 +308,16 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, help="only run the counter-calibration copy of this size")
-    ap.add_argument("--workload", choices=["dense", "batch"], default="dense")
+    ap.add_argument("--workload", choices=["dense", "batch", "learned"], default="dense")
     ap.add_argument("--samples", type=int, default=256)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
+    ap.add_argument("--lp-burn", type=int, default=8192)
+    ap.add_argument("--lp-snapshots", type=int, default=8192)
+    ap.add_argument("--lp-train-steps", type=int, default=1500)
+    ap.add_argument("--lp-scale", type=float, default=0.2)
+    ap.add_argument("--lp-every", type=int, default=8)
+    ap.add_argument("--lp-blocks", type=int, default=4)
+    ap.add_argument("--lp-block", type=int, default=16384)
     args = ap.parse_args()
     if args.copy_calib:
         copy_calibration(args.copy_calib)
@@ -212,6 +329,9 @@ def main():
 
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "learned":
+        print(json.dumps({"learned_vs_plain": learned_vs_plain(args)}))
         return
 
     import cuda_pspin_amd as P
@@ -251,7 +371,7 @@ def main():
         bpu = 2.0 * st.info.n_site_entries / N * 8 + (2 * N * 4 if st.info.h2_active else 0)
     achieved = pu_per_step * bpu / (kern_ms * 1e-3)
     out = {
-        "metric": f"pair-updates/s (dense PT, N={N}, 1 MI355X)",
+        "metric": "pair-updates/s + energy ESS/s at T_min, N=64 PT: learned vs plain, 1 MI355X",
         "value": value,
         "unit": "pair-updates/s",
         "n_gpus": 1,
@@ -279,6 +399,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(st, N, nrep, np.repeat(beta, nr), args.seed, args.cpu_sweeps)
     st.close()
+    if not args.no_learned:
+        out["learned_vs_plain"] = learned_vs_plain(args)
     print(json.dumps(out))
 
 
