@@ -117,6 +117,8 @@ def learned_vs_plain(args):
     R = NT * nr
     T = np.array([args.Tmax * (args.Tmin / args.Tmax) ** (t / (NT - 1)) for t in range(NT)])
     st = P.PSpin(P.SimConfig.default(N=N, nrep=R, seed=args.seed, fmc_mode=2, gamma=0.05, dev=0), P.DENSE)
+    if args.no_graphs:
+        st.set_profiling(True)  # per-launch events instead of captured graphs
     st.pt_init(NT, nr, 1.0 / T, args.seed + 5000)
     st.pt_run(args.lp_burn, 1)
     lcfg = P.LPConfig.default(hidden=512, batch_cap=4096, data_cap=args.lp_snapshots, seed=args.seed)
@@ -311,6 +313,7 @@ def main():
     ap.add_argument("--workload", choices=["dense", "batch", "learned"], default="dense")
     ap.add_argument("--samples", type=int, default=256)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
+    ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (for tracing tools)")
     ap.add_argument("--lp-burn", type=int, default=8192)
     ap.add_argument("--lp-snapshots", type=int, default=8192)
     ap.add_argument("--lp-train-steps", type=int, default=1500)

@@ -559,9 +559,13 @@ int pspin_lp_train(pspin_learned* lp, long long n_steps, int batch, int lag, dou
 
 int pspin_lp_move(pspin_learned* lp) {
     LP_REQUIRE(lp, PSPIN_ERR_ARG, "null handle");
-    int rc = ensure_graph(lp, 1, 0);
-    if (rc) return rc;
-    LP_TRY_HIP(hipGraphLaunch(lp->graph, lp->st->stream));
+    if (lp->st->prof || lp->st->d_log) {  // per-launch recording: no graph
+        LP_TRY_HIP(move_launch(lp, move_args(lp)));
+    } else {
+        int rc = ensure_graph(lp, 1, 0);
+        if (rc) return rc;
+        LP_TRY_HIP(hipGraphLaunch(lp->graph, lp->st->stream));
+    }
     LP_TRY_HIP(sync(lp));
     lp->st->since_resync = 0;  // the move leaves every running energy at its FP64 value
     return PSPIN_OK;
