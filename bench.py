@@ -268,6 +268,7 @@ def run_batch(args):
     t_init = time.time() - t0
     b.set_resync_interval(1024)
     b.run(args.warmup, 1)
+    chunk, trial_ms = b.autotune(512, 1)  # sweeps per launch; restores the batch afterwards
     t0 = time.perf_counter()
     b.run(args.steps, 1)
     wall = time.perf_counter() - t0
@@ -282,7 +283,8 @@ def run_batch(args):
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded disorder and spins)",
            "config": {"workload": "batched sparse cube PT: 256 samples N in {24,32,48,64}, fmc uniform 0.05, "
                                   "NT=24 x nrep=4, exchange every sweep, resync 1024",
-                      "samples": n_samples, "chains": b.chains, "N_hist": {str(n): Ns.count(n) for n in (24, 32, 48, 64)}},
+                      "samples": n_samples, "chains": b.chains, "N_hist": {str(n): Ns.count(n) for n in (24, 32, 48, 64)},
+                      "autotuned_sweeps_per_launch": chunk, "autotune_trial_ms_per_sweep": trial_ms},
            "init_s": t_init,
            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK, "traffic": load_traffic_key("batch_256"),
@@ -348,9 +350,13 @@ def main():
     cfg = P.SimConfig.default(N=N, nrep=nrep, seed=args.seed, fmc_mode=args.fmc, gamma=args.gamma, dev=0)
     t0 = time.time()
     st = P.PSpin(cfg, P.DENSE)
-    st.set_kernel(args.kernel, args.block)
     if args.coupling == "bf16":
         st.set_coupling_format(1)
+    tuned = None
+    if args.kernel == 0 and args.block == 0:
+        tuned = st.autotune(8)  # times every kernel / tile candidate on this state
+    else:
+        st.set_kernel(args.kernel, args.block)
     t_init = time.time() - t0
     st.pt_init(NT, nr, beta, args.seed + 5000)
     st.pt_run(args.warmup, 1)
@@ -389,7 +395,9 @@ def main():
         "config": {"workload": f"dense spherical PT N={N} fmc={args.fmc} NT={NT} x nrep={nr} "
                                f"T in [{Tmin},{Tmax}] geometric, exchange every sweep",
                    "N": N, "NT": NT, "nrep": nr, "replicas": nrep, "pt_freq": 1, "fmc": args.fmc,
-                   "sweeps_timed": args.steps, "kernel": st.info.kernel, "coupling": args.coupling},
+                   "sweeps_timed": args.steps, "kernel": st.info.kernel, "coupling": args.coupling,
+                   "autotuned": None if tuned is None else {"kernel": tuned[0], "block": tuned[1],
+                                                             "trial_ms_per_sweep": tuned[2]}},
         "energy_ess_per_s_tmin": ess / wall,
         "energy_tau_int_tmin_sweeps": float(np.nanmean(taus)) if taus else None,
         "sweep_kernel_ms": kern_ms,

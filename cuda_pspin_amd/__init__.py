@@ -135,6 +135,8 @@ def load_library() -> C.CDLL:
         "pspin_get_profile": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
         "pspin_set_kernel": (C.c_int, [P, C.c_int, C.c_int]),
+        "pspin_autotune": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double)]),
         "pspin_set_resync_interval": (C.c_int, [P, C.c_int]),
         "pspin_set_coupling_format": (C.c_int, [P, C.c_int]),
         "pspin_batch_create": (C.c_int, [P, C.c_int, i32p, u64p, C.c_int, C.c_int, dp, C.POINTER(P)]),
@@ -146,6 +148,9 @@ def load_library() -> C.CDLL:
         "pspin_batch_get_spins": (C.c_int, [P, C.c_int, dp]),
         "pspin_batch_get_perm": (C.c_int, [P, C.c_int, i32p]),
         "pspin_batch_get_exchange_stats": (C.c_int, [P, C.c_int, i64p, i64p]),
+        "pspin_batch_set_chunk": (C.c_int, [P, C.c_int]),
+        "pspin_batch_autotune": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_double)]),
         "pspin_lp_default_config": (C.c_int, [P]),
         "pspin_lp_create": (C.c_int, [P, P, C.POINTER(P)]),
         "pspin_lp_destroy": (C.c_int, [P]),
@@ -364,6 +369,16 @@ class PSpin:
     def set_resync_interval(self, sweeps: int) -> None:
         _check(self._L.pspin_set_resync_interval(self._h, int(sweeps)))
 
+    def autotune(self, trial_sweeps: int = 8, verbose: bool = False):
+        """Time the sweep kernel candidates on the live state; keep the fastest.
+        Returns (kernel, block_size, ms_per_sweep)."""
+        k, b, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(self._L.pspin_autotune(self._h, int(trial_sweeps), int(verbose), C.byref(k), C.byref(b), C.byref(ms)))
+        info = Info()
+        _check(self._L.pspin_get_info(self._h, C.byref(info)))
+        self.info = info
+        return k.value, b.value, ms.value
+
     def set_kernel(self, kernel: int = KERNEL_AUTO, block_size: int = 0) -> None:
         _check(self._L.pspin_set_kernel(self._h, int(kernel), int(block_size)))
         info = Info()
@@ -408,6 +423,16 @@ class PSpinBatch:
 
     def run(self, n_sweeps: int, pt_freq: int = 1):
         _check(self._L.pspin_batch_run(self._h, int(n_sweeps), int(pt_freq)))
+
+    def set_chunk(self, sweeps_per_launch: int):
+        _check(self._L.pspin_batch_set_chunk(self._h, int(sweeps_per_launch)))
+
+    def autotune(self, trial_sweeps: int = 256, pt_freq: int = 1, verbose: bool = False):
+        """Time sweeps-per-launch candidates; keep the fastest. Returns (chunk, ms_per_sweep)."""
+        c, ms = C.c_int(), C.c_double()
+        _check(self._L.pspin_batch_autotune(self._h, int(trial_sweeps), int(pt_freq), int(verbose), C.byref(c),
+                                            C.byref(ms)))
+        return c.value, ms.value
 
     def results(self, s: int):
         c = self.chains_per_sample

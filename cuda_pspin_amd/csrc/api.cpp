@@ -135,6 +135,46 @@ int build_site_lists(pspin_state* st, const std::vector<Term>& terms) {
     return PSPIN_OK;
 }
 
+// Work list of the pair-row kernel for `nw` waves: blocks of 64 chunks of 4
+// pairs, split into ranges of 8 rows a < b_max(block) for both sites of the
+// step, dealt longest first to the least loaded wave.
+int build_pair_tasks(pspin_state* st, int nw_req) {
+    const std::vector<uint32_t>& pbc = st->h_pair_bc;
+    const int npairs = (int)pbc.size();
+    const int nchunks = (npairs + 3) / 4, nblocks = (nchunks + 63) / 64;
+    constexpr int kRows = 8;
+    std::vector<std::array<int, 4>> tasks;
+    for (int slot = 0; slot < 2; slot++)
+        for (int blk = 0; blk < nblocks; blk++) {
+            const int bmax = (int)(pbc[(size_t)blk * 256] & 0xffff);  // rows a < bmax
+            for (int a0 = 0; a0 < bmax; a0 += kRows) tasks.push_back({slot, blk * 64, a0, std::min(bmax, a0 + kRows)});
+        }
+    const int nw = std::max(1, std::min<int>(nw_req, (int)tasks.size()));
+    std::stable_sort(tasks.begin(), tasks.end(),
+                     [](const std::array<int, 4>& p, const std::array<int, 4>& q) { return p[3] - p[2] > q[3] - q[2]; });
+    std::vector<std::vector<std::array<int, 4>>> per(nw);
+    std::vector<int> load(nw, 0);
+    for (const auto& t : tasks) {
+        int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        per[w].push_back(t);
+        load[w] += (t[3] - t[2]) + 2;  // rows plus per-task set-up
+    }
+    std::vector<int4> flat;
+    std::vector<int> wt(nw + 1, 0);
+    for (int w = 0; w < nw; w++) {
+        for (const auto& t : per[w]) flat.push_back(make_int4(t[0], t[1], t[2], t[3]));
+        wt[w + 1] = (int)flat.size();
+    }
+    dfree(st->d_tasks);
+    dfree(st->d_wave_tasks);
+    PS_TRY_HIP(dmalloc(&st->d_tasks, flat.size()));
+    PS_TRY_HIP(dmalloc(&st->d_wave_tasks, wt.size()));
+    PS_TRY_HIP(cpy(st, st->d_tasks, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
+    PS_TRY_HIP(cpy(st, st->d_wave_tasks, wt.data(), wt.size() * sizeof(int), hipMemcpyHostToDevice));
+    st->pair_nwaves = nw;
+    return PSPIN_OK;
+}
+
 // Dense per-site pair-row slices for the fully connected case (every quartet
 // channel 0).  Others of site x are relabelled 0..M-1 (M = N-1, o(y) = y + (y >= x)).
 // Pairs (b<c) are ordered by b descending, so the pairs with b > a -- row a --
@@ -173,45 +213,16 @@ int build_pair_layout(pspin_state* st, const std::vector<double>& g4, const std:
             }
         }
     }
-    // tasks: blocks of 64 chunks of 4 pairs; rows [0, b_max(block))
-    const int nchunks = (npairs + 3) / 4, nblocks = (nchunks + 63) / 64;
-    constexpr int kRows = 8;
-    std::vector<std::array<int, 4>> tasks;
-    for (int slot = 0; slot < 2; slot++)
-        for (int blk = 0; blk < nblocks; blk++) {
-            const int bmax = (int)(pbc[(size_t)blk * 256] & 0xffff);  // rows a < bmax
-            for (int a0 = 0; a0 < bmax; a0 += kRows) tasks.push_back({slot, blk * 64, a0, std::min(bmax, a0 + kRows)});
-        }
-    const int nw = std::max(1, std::min<int>(16, (int)tasks.size()));
-    std::stable_sort(tasks.begin(), tasks.end(),
-                     [](const std::array<int, 4>& p, const std::array<int, 4>& q) { return p[3] - p[2] > q[3] - q[2]; });
-    std::vector<std::vector<std::array<int, 4>>> per(nw);
-    std::vector<int> load(nw, 0);
-    for (const auto& t : tasks) {
-        int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        per[w].push_back(t);
-        load[w] += (t[3] - t[2]) + 2;  // rows plus per-task set-up
-    }
-    std::vector<int4> flat;
-    std::vector<int> wt(nw + 1, 0);
-    for (int w = 0; w < nw; w++) {
-        for (const auto& t : per[w]) flat.push_back(make_int4(t[0], t[1], t[2], t[3]));
-        wt[w + 1] = (int)flat.size();
-    }
     st->pair_slice_len = slice;
     st->pair_npairs = npairs;
-    st->pair_nwaves = nw;
     PS_TRY_HIP(dmalloc(&st->d_ptab, tab.size()));
     PS_TRY_HIP(dmalloc(&st->d_pair_bc, pbc.size()));
-    PS_TRY_HIP(dmalloc(&st->d_tasks, flat.size()));
-    PS_TRY_HIP(dmalloc(&st->d_wave_tasks, wt.size()));
     PS_TRY_HIP(cpy(st, st->d_ptab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
     st->h_row_off = roff;
     st->h_row_len = rlen;
+    st->h_pair_bc = pbc;
     PS_TRY_HIP(cpy(st, st->d_pair_bc, pbc.data(), pbc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    PS_TRY_HIP(cpy(st, st->d_tasks, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
-    PS_TRY_HIP(cpy(st, st->d_wave_tasks, wt.data(), wt.size() * sizeof(int), hipMemcpyHostToDevice));
-    return PSPIN_OK;
+    return build_pair_tasks(st, 16);
 }
 
 // splitmix64 frequencies (src/mc.cu:449-460).
@@ -1047,6 +1058,121 @@ int pspin_device_info(int dev, size_t* free_bytes, size_t* total_bytes, char* na
         PS_TRY_HIP(hipGetDeviceProperties(&prop, dev >= 0 ? dev : cur));
         std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
     }
+    return PSPIN_OK;
+}
+
+// Sweep autotuner (the reference picks a block size from the occupancy API
+// alone, src/mc.cu:709-744, and prints an [auto-tune] line).  This one times
+// every candidate on the live state -- the list kernel at 64..1024 threads and,
+// for fully connected systems, the pair-row kernel at 4..16 waves -- and
+// restores spins, energies, counters and RNG counters after each trial, so a
+// call leaves the chain exactly as it found it.
+int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, int* block, double* ms_per_sweep) {
+    PS_REQUIRE(st && trial_sweeps >= 1, PSPIN_ERR_ARG, "bad argument");
+    PS_REQUIRE(!st->d_log, PSPIN_ERR_STATE, "disable the step log before autotuning");
+    const size_t ns = (size_t)st->nrep * 2 * st->N;
+    double *sv = nullptr, *ev = nullptr;
+    long long *av = nullptr, *pv = nullptr;
+    uint64_t* cv = nullptr;
+    auto cleanup = [&]() { dfree(sv); dfree(ev); dfree(av); dfree(pv); dfree(cv); };
+    hipError_t e = dmalloc(&sv, ns);
+    if (!e) e = dmalloc(&ev, st->nrep);
+    if (!e) e = dmalloc(&av, st->nrep);
+    if (!e) e = dmalloc(&pv, st->nrep);
+    if (!e) e = dmalloc(&cv, st->nrep);
+    auto save = [&](bool to_saved) {
+        auto c = [&](void* d, const void* s2, size_t b) { return hipMemcpyAsync(d, s2, b, hipMemcpyDeviceToDevice, st->stream); };
+        hipError_t r = hipSuccess;
+        if (to_saved) {
+            if (!r) r = c(sv, st->d_spins, ns * 8);
+            if (!r) r = c(ev, st->d_E, st->nrep * 8);
+            if (!r) r = c(av, st->d_acc, st->nrep * 8);
+            if (!r) r = c(pv, st->d_prop, st->nrep * 8);
+            if (!r) r = c(cv, st->d_ctr, st->nrep * 8);
+        } else {
+            if (!r) r = c(st->d_spins, sv, ns * 8);
+            if (!r) r = c(st->d_E, ev, st->nrep * 8);
+            if (!r) r = c(st->d_acc, av, st->nrep * 8);
+            if (!r) r = c(st->d_prop, pv, st->nrep * 8);
+            if (!r) r = c(st->d_ctr, cv, st->nrep * 8);
+        }
+        return r;
+    };
+    if (!e) e = save(true);
+    if (e) {
+        cleanup();
+        ps::set_error(std::string("HIP error in pspin_autotune: ") + hipGetErrorString(e));
+        return PSPIN_ERR_HIP;
+    }
+    struct Cand { int kernel, param; };
+    std::vector<Cand> cands;
+    const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
+    if (st->d_ptab && !bf16 && st->variant == PSPIN_DENSE)
+        for (int nw : {4, 8, 12, 16}) cands.push_back({PSPIN_KERNEL_PAIRS, nw});
+    for (int bs : {64, 128, 256, 512, 1024}) cands.push_back({PSPIN_KERNEL_LISTS, bs});
+    const int old_kernel = st->kernel, old_block = st->block_size, old_nw = st->pair_nwaves;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    double best = 1e300;
+    Cand pick = {old_kernel, old_kernel == PSPIN_KERNEL_PAIRS ? old_nw : old_block};
+    int rc = PSPIN_OK;
+    for (const Cand& c : cands) {
+        if (c.kernel == PSPIN_KERNEL_PAIRS) {
+            if ((rc = build_pair_tasks(st, c.param))) break;
+        } else {
+            st->block_size = c.param;
+        }
+        st->kernel = c.kernel;
+        hipError_t r = ps::launch_sweep(st, 1);  // warm-up
+        if (!r) r = hipEventRecord(e0, st->stream);
+        if (!r) r = ps::launch_sweep(st, trial_sweeps);
+        if (!r) r = hipEventRecord(e1, st->stream);
+        if (!r) r = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (!r) r = hipEventElapsedTime(&ms, e0, e1);
+        if (!r) r = save(false);
+        if (r) {
+            ps::set_error(std::string("HIP error in pspin_autotune: ") + hipGetErrorString(r));
+            rc = PSPIN_ERR_HIP;
+            break;
+        }
+        const double per = ms / trial_sweeps;
+        if (verbose)
+            std::printf("[auto-tune] sweep N=%d nrep=%d %s %s=%d: %.4f ms/sweep\n", st->N, st->nrep,
+                        c.kernel == PSPIN_KERNEL_PAIRS ? "pair-row" : "list", c.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block",
+                        c.param, per);
+        if (per < best) {
+            best = per;
+            pick = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == PSPIN_OK) {
+        st->kernel = pick.kernel;
+        if (pick.kernel == PSPIN_KERNEL_PAIRS) {
+            rc = build_pair_tasks(st, pick.param);
+        } else {
+            st->block_size = pick.param;
+            if (st->d_ptab && st->pair_nwaves != old_nw) rc = build_pair_tasks(st, old_nw);
+        }
+    } else {
+        st->kernel = old_kernel;
+        st->block_size = old_block;
+        if (st->d_ptab) (void)build_pair_tasks(st, old_nw);
+    }
+    (void)hipStreamSynchronize(st->stream);
+    cleanup();
+    drop_graph(st);
+    if (rc) return rc;
+    if (verbose)
+        std::printf("[auto-tune] sweep N=%d -> %s %s=%d (%.4f ms/sweep)\n", st->N,
+                    pick.kernel == PSPIN_KERNEL_PAIRS ? "pair-row" : "list",
+                    pick.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block", pick.param, best);
+    if (kernel) *kernel = pick.kernel;
+    if (block) *block = pick.kernel == PSPIN_KERNEL_PAIRS ? 64 * pick.param : pick.param;
+    if (ms_per_sweep) *ms_per_sweep = best;
     return PSPIN_OK;
 }
 

@@ -50,6 +50,8 @@ struct pspin_batch {
     long long pair_updates_per_sweep = 0;
     hipGraphExec_t graph = nullptr;
     int graph_sweeps = 0, graph_freq = 0;
+    int chunk = 64;  // sweeps per launch (autotuned; results do not depend on it)
+    size_t n_spin = 0, n_perm = 0, n_ex = 0;
 };
 
 namespace {
@@ -261,6 +263,9 @@ int pspin_batch_create(const pspin_sim_config* base, int n_samples, const int* N
     PB_ALLOC(b->d_since, nch, zero32.data());
     PB_ALLOC(b->d_perm, perm.size(), perm.data());
 #undef PB_ALLOC
+    b->n_spin = spins.size();
+    b->n_perm = perm.size();
+    b->n_ex = (size_t)n_samples * 4 * (NT - 1);
     if (ps::batch_prepare(b->lds) != hipSuccess) {
         ps::set_error("the batch kernel cannot get " + std::to_string(b->lds) + " bytes of LDS");
         return fail(PSPIN_ERR_HIP);
@@ -285,25 +290,30 @@ int pspin_batch_set_resync_interval(pspin_batch* b, int sweeps) {
     return PSPIN_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
 // n_sweeps sweeps of every chain; an exchange for every sample after each
 // sweep that completes a multiple of pt_freq (0 = no exchange).  Launches of
-// kBatchChunk sweeps are replayed from a captured graph.
-int pspin_batch_run(pspin_batch* b, int n_sweeps, int pt_freq) {
-    PB_REQUIRE(b && n_sweeps >= 0 && pt_freq >= 0, PSPIN_ERR_ARG, "bad argument");
-    constexpr int kBatchChunk = 64, kGraphLaunches = 8;
+// b->chunk sweeps are replayed from a captured graph of ~512 sweeps.  All
+// chain state (RNG and resync counters, permutation, drand48 state) lives in
+// device memory, so the result does not depend on the launch split.
+int batch_advance(pspin_batch* b, int n_sweeps, int pt_freq) {
+    const int chunk = b->chunk, launches = std::max(1, 512 / chunk);
     ps::BatchArgs a;
     int done = 0;
     while (done < n_sweeps) {
         const int left = n_sweeps - done;
-        if (left >= kBatchChunk * kGraphLaunches) {
-            if (!b->graph || b->graph_freq != pt_freq) {
+        if (left >= chunk * launches) {
+            if (!b->graph || b->graph_freq != pt_freq || b->graph_sweeps != chunk) {
                 if (b->graph) (void)hipGraphExecDestroy(b->graph);
                 b->graph = nullptr;
-                batch_args(b, a, kBatchChunk, pt_freq);
+                batch_args(b, a, chunk, pt_freq);
                 hipGraph_t g = nullptr;
                 PB_TRY_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
                 hipError_t e = hipSuccess;
-                for (int k = 0; k < kGraphLaunches && e == hipSuccess; k++)
+                for (int k = 0; k < launches && e == hipSuccess; k++)
                     e = ps::launch_batch(a, b->n, b->threads, b->lds, b->stream);
                 hipError_t e2 = hipStreamEndCapture(b->stream, &g);
                 PB_TRY_HIP(e);
@@ -312,17 +322,102 @@ int pspin_batch_run(pspin_batch* b, int n_sweeps, int pt_freq) {
                 (void)hipGraphDestroy(g);
                 PB_TRY_HIP(e3);
                 b->graph_freq = pt_freq;
+                b->graph_sweeps = chunk;
             }
             PB_TRY_HIP(hipGraphLaunch(b->graph, b->stream));
-            done += kBatchChunk * kGraphLaunches;
+            done += chunk * launches;
             continue;
         }
-        const int chunk = std::min(left, kBatchChunk);
-        batch_args(b, a, chunk, pt_freq);
+        const int c = std::min(left, chunk);
+        batch_args(b, a, c, pt_freq);
         PB_TRY_HIP(ps::launch_batch(a, b->n, b->threads, b->lds, b->stream));
-        done += chunk;
+        done += c;
     }
+    return PSPIN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pspin_batch_run(pspin_batch* b, int n_sweeps, int pt_freq) {
+    PB_REQUIRE(b && n_sweeps >= 0 && pt_freq >= 0, PSPIN_ERR_ARG, "bad argument");
+    const int rc = batch_advance(b, n_sweeps, pt_freq);
+    if (rc) return rc;
     PB_TRY_HIP(hipStreamSynchronize(b->stream));
+    return PSPIN_OK;
+}
+
+int pspin_batch_set_chunk(pspin_batch* b, int sweeps_per_launch) {
+    PB_REQUIRE(b && sweeps_per_launch >= 1 && sweeps_per_launch <= 4096, PSPIN_ERR_ARG, "bad argument");
+    b->chunk = sweeps_per_launch;
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    return PSPIN_OK;
+}
+
+// Times sweeps-per-launch candidates on the live batch and keeps the fastest;
+// every mutable device array is saved first and restored after each trial.
+int pspin_batch_autotune(pspin_batch* b, int trial_sweeps, int pt_freq, int verbose, int* chunk_out,
+                         double* ms_per_sweep) {
+    PB_REQUIRE(b && trial_sweeps >= 1 && pt_freq >= 0, PSPIN_ERR_ARG, "bad argument");
+    const long long nch = b->chains;
+    struct Buf { void* dev; size_t bytes; void* saved; };
+    std::vector<Buf> bufs = {{b->d_spins, b->n_spin * 8, nullptr}, {b->d_E, (size_t)nch * 8, nullptr},
+                             {b->d_betas, (size_t)nch * 8, nullptr}, {b->d_acc, (size_t)nch * 8, nullptr},
+                             {b->d_prop, (size_t)nch * 8, nullptr}, {b->d_ex, b->n_ex * 8, nullptr},
+                             {b->d_done, (size_t)b->n * 8, nullptr}, {b->d_ctr, (size_t)nch * 8, nullptr},
+                             {b->d_rng48, (size_t)b->n * 8, nullptr}, {b->d_since, (size_t)nch * 4, nullptr},
+                             {b->d_perm, b->n_perm * 4, nullptr}};
+    int rc = PSPIN_OK;
+    auto fail = [&](hipError_t e) {
+        ps::set_error(std::string("HIP error in pspin_batch_autotune: ") + hipGetErrorString(e));
+        rc = PSPIN_ERR_HIP;
+    };
+    for (Buf& x : bufs) {
+        hipError_t e = hipMalloc(&x.saved, std::max<size_t>(x.bytes, 1));
+        if (!e) e = hipMemcpyAsync(x.saved, x.dev, x.bytes, hipMemcpyDeviceToDevice, b->stream);
+        if (e) { fail(e); break; }
+    }
+    const int old = b->chunk;
+    int best_chunk = old;
+    double best = 1e300;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    for (int c : {16, 32, 64, 128, 256}) {
+        if (rc) break;
+        b->chunk = c;
+        if (b->graph) (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        hipError_t e = hipEventRecord(e0, b->stream);
+        if (!e && (rc = batch_advance(b, trial_sweeps, pt_freq))) break;
+        if (!e) e = hipEventRecord(e1, b->stream);
+        if (!e) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (!e) e = hipEventElapsedTime(&ms, e0, e1);
+        for (Buf& x : bufs)
+            if (!e) e = hipMemcpyAsync(x.dev, x.saved, x.bytes, hipMemcpyDeviceToDevice, b->stream);
+        if (e) { fail(e); break; }
+        const double per = ms / trial_sweeps;
+        if (verbose) std::printf("[auto-tune] batch %d samples: %d sweeps/launch: %.4f ms/sweep\n", b->n, c, per);
+        if (per < best) {
+            best = per;
+            best_chunk = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(b->stream);
+    for (Buf& x : bufs)
+        if (x.saved) (void)hipFree(x.saved);
+    b->chunk = rc ? old : best_chunk;
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    if (rc) return rc;
+    if (verbose) std::printf("[auto-tune] batch -> %d sweeps/launch (%.4f ms/sweep)\n", best_chunk, best);
+    if (chunk_out) *chunk_out = best_chunk;
+    if (ms_per_sweep) *ms_per_sweep = best;
     return PSPIN_OK;
 }
 

@@ -190,6 +190,7 @@ struct pspin_state {
     std::vector<int> h_row_off;   // [N-3]
     std::vector<int> h_row_len;   // [N-3]
     uint32_t* d_pair_bc = nullptr;
+    std::vector<uint32_t> h_pair_bc;
     int pair_npairs = 0;
     int4* d_tasks = nullptr;
     int* d_wave_tasks = nullptr;

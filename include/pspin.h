@@ -195,6 +195,11 @@ int pspin_batch_get_results(const pspin_batch* b, int sample, double* E, long lo
 int pspin_batch_get_spins(const pspin_batch* b, int sample, double* re_im);
 int pspin_batch_get_perm(const pspin_batch* b, int sample, int* perm);
 int pspin_batch_get_exchange_stats(const pspin_batch* b, int sample, long long* acc_tot, long long* prop_tot);
+/* Sweeps per kernel launch (default 64; results do not depend on it), and a
+ * timing autotuner over 16..256 that restores the batch after each trial. */
+int pspin_batch_set_chunk(pspin_batch* b, int sweeps_per_launch);
+int pspin_batch_autotune(pspin_batch* b, int trial_sweeps, int pt_freq, int verbose, int* chunk,
+                         double* ms_per_sweep);
 
 /* ---------------- test and profiling hooks (no reference counterpart) --- */
 /* Per-step log of the next sweeps: capacity_sweeps sweeps per replica.
@@ -225,6 +230,12 @@ int pspin_device_info(int dev, size_t* free_bytes, size_t* total_bytes, char* na
 #define PSPIN_COUPLING_F32 0
 #define PSPIN_COUPLING_BF16X2 1 /* BF16 hi/lo split, FP32 accumulation       */
 int pspin_set_kernel(pspin_state* st, int kernel, int block_size);
+/* Timing autotuner for the sweep (replaces optimal_block_size_for_sweep,
+ * src/mc.cu:709-744, which only queries occupancy): times each candidate
+ * kernel / tile shape for trial_sweeps sweeps on the live state, restores the
+ * state after every trial, keeps the fastest.  verbose prints one
+ * "[auto-tune]" line per candidate.  Outputs may be NULL. */
+int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, int* block, double* ms_per_sweep);
 /* Coupling storage of the list kernel: FP32 (default) or BF16 hi/lo pairs
  * (g ~ hi + lo, ~16 mantissa bits) with FP32 accumulation. */
 int pspin_set_coupling_format(pspin_state* st, int format);

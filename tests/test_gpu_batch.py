@@ -78,3 +78,48 @@ def test_batch_energy_constraint_and_resync(pspin, oracle):
             assert abs(E[c] - (e2 + e4)) <= 1e-9 * max(1.0, abs(e2 + e4))
         st.close()
     b.close()
+
+
+def _batch_state(b, n):
+    out = []
+    for s in range(n):
+        E, a, p = b.results(s)
+        out.append((b.spins(s), E, a, p, b.perm(s), *b.exchange_stats(s)))
+    return out
+
+
+def _same(x, y):
+    for u, v in zip(x, y):
+        for p, q in zip(u, v):
+            np.testing.assert_array_equal(p, q)
+
+
+def test_batch_result_independent_of_launch_split_and_autotune(pspin, oracle):
+    """Sweeps per launch is a pure launch-shape choice: every chain's state is
+    device-resident, so any split gives the same bits; the autotuner restores
+    the batch after its trials."""
+    Ns, seeds = [16, 32, 24, 48], [7, 8, 9, 10]
+    NT, nr = 4, 2
+    T, beta = oracle.temperatures(NT, 0.3, 1.6, 1)
+    base = pspin.SimConfig.default(fmc_mode=2, gamma=0.1)
+    runs = []
+    for chunk in (16, 64, 256):
+        b = pspin.PSpinBatch(base, Ns, seeds, NT, nr, beta)
+        b.set_resync_interval(100)
+        b.set_chunk(chunk)
+        b.run(700, 1)  # graph replays plus a remainder launch, resyncs at 100, 200, ...
+        runs.append(_batch_state(b, len(Ns)))
+        b.close()
+    _same(runs[0], runs[1])
+    _same(runs[0], runs[2])
+    b = pspin.PSpinBatch(base, Ns, seeds, NT, nr, beta)
+    b.set_resync_interval(100)
+    b.run(300, 1)
+    before = _batch_state(b, len(Ns))
+    chunk, ms = b.autotune(256, 1)
+    print(f"  autotuned sweeps/launch {chunk}: {ms:.4f} ms/sweep")
+    assert chunk in (16, 32, 64, 128, 256) and ms > 0
+    _same(before, _batch_state(b, len(Ns)))
+    b.run(400, 1)
+    _same(runs[0], _batch_state(b, len(Ns)))  # 300 + 400 = 700 sweeps, same bits
+    b.close()
