@@ -316,6 +316,9 @@ def main():
     ap.add_argument("--samples", type=int, default=256)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (for tracing tools)")
+    ap.add_argument("--pair-groups", type=int, default=8,
+                    help="replicas r = g (mod G) share a pair stream, one group per XCD and per ladder "
+                         "(0: per-replica pairs, as the reference)")
     ap.add_argument("--lp-burn", type=int, default=8192)
     ap.add_argument("--lp-snapshots", type=int, default=8192)
     ap.add_argument("--lp-train-steps", type=int, default=1500)
@@ -352,6 +355,7 @@ def main():
     st = P.PSpin(cfg, P.DENSE)
     if args.coupling == "bf16":
         st.set_coupling_format(1)
+    st.set_pair_schedule(args.pair_groups)
     tuned = None
     if args.kernel == 0 and args.block == 0:
         tuned = st.autotune(8)  # times every kernel / tile candidate on this state
@@ -396,6 +400,7 @@ def main():
                                f"T in [{Tmin},{Tmax}] geometric, exchange every sweep",
                    "N": N, "NT": NT, "nrep": nr, "replicas": nrep, "pt_freq": 1, "fmc": args.fmc,
                    "sweeps_timed": args.steps, "kernel": st.info.kernel, "coupling": args.coupling,
+                   "pair_groups": args.pair_groups,
                    "autotuned": None if tuned is None else {"kernel": tuned[0], "block": tuned[1],
                                                              "trial_ms_per_sweep": tuned[2]}},
         "energy_ess_per_s_tmin": ess / wall,

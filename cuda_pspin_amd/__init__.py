@@ -135,6 +135,10 @@ def load_library() -> C.CDLL:
         "pspin_get_profile": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
         "pspin_set_kernel": (C.c_int, [P, C.c_int, C.c_int]),
+        "pspin_set_pair_schedule": (C.c_int, [P, C.c_int]),
+        "pspin_set_pair_tiles": (C.c_int, [P, C.c_int, C.c_int]),
+        "pspin_set_list_batch": (C.c_int, [P, C.c_int]),
+        "pspin_get_pair_keys": (C.c_int, [P, u64p]),
         "pspin_autotune": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_double)]),
         "pspin_set_resync_interval": (C.c_int, [P, C.c_int]),
@@ -368,6 +372,21 @@ class PSpin:
 
     def set_resync_interval(self, sweeps: int) -> None:
         _check(self._L.pspin_set_resync_interval(self._h, int(sweeps)))
+
+    def set_pair_schedule(self, groups: int) -> None:
+        """0: per-replica pairs; G > 0: replicas r = g (mod G) share a pair stream."""
+        _check(self._L.pspin_set_pair_schedule(self._h, int(groups)))
+
+    def set_pair_tiles(self, nwaves: int, rows: int) -> None:
+        _check(self._L.pspin_set_pair_tiles(self._h, int(nwaves), int(rows)))
+
+    def set_list_batch(self, batch: int) -> None:
+        _check(self._L.pspin_set_list_batch(self._h, int(batch)))
+
+    def pair_keys(self) -> np.ndarray:
+        k = np.zeros(self.nrep, dtype=np.uint64)
+        _check(self._L.pspin_get_pair_keys(self._h, k))
+        return k
 
     def autotune(self, trial_sweeps: int = 8, verbose: bool = False):
         """Time the sweep kernel candidates on the live state; keep the fastest.

@@ -138,11 +138,11 @@ int build_site_lists(pspin_state* st, const std::vector<Term>& terms) {
 // Work list of the pair-row kernel for `nw` waves: blocks of 64 chunks of 4
 // pairs, split into ranges of 8 rows a < b_max(block) for both sites of the
 // step, dealt longest first to the least loaded wave.
-int build_pair_tasks(pspin_state* st, int nw_req) {
+int build_pair_tasks(pspin_state* st, int nw_req, int rows) {
     const std::vector<uint32_t>& pbc = st->h_pair_bc;
     const int npairs = (int)pbc.size();
     const int nchunks = (npairs + 3) / 4, nblocks = (nchunks + 63) / 64;
-    constexpr int kRows = 8;
+    const int kRows = rows == 16 ? 16 : 8;
     std::vector<std::array<int, 4>> tasks;
     for (int slot = 0; slot < 2; slot++)
         for (int blk = 0; blk < nblocks; blk++) {
@@ -172,6 +172,7 @@ int build_pair_tasks(pspin_state* st, int nw_req) {
     PS_TRY_HIP(cpy(st, st->d_tasks, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
     PS_TRY_HIP(cpy(st, st->d_wave_tasks, wt.data(), wt.size() * sizeof(int), hipMemcpyHostToDevice));
     st->pair_nwaves = nw;
+    st->pair_rows = kRows;
     return PSPIN_OK;
 }
 
@@ -222,7 +223,7 @@ int build_pair_layout(pspin_state* st, const std::vector<double>& g4, const std:
     st->h_row_len = rlen;
     st->h_pair_bc = pbc;
     PS_TRY_HIP(cpy(st, st->d_pair_bc, pbc.data(), pbc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    return build_pair_tasks(st, 16);
+    return build_pair_tasks(st, 16, 8);
 }
 
 // splitmix64 frequencies (src/mc.cu:449-460).
@@ -1104,24 +1105,31 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
         ps::set_error(std::string("HIP error in pspin_autotune: ") + hipGetErrorString(e));
         return PSPIN_ERR_HIP;
     }
-    struct Cand { int kernel, param; };
+    struct Cand { int kernel, param, rows; };
     std::vector<Cand> cands;
     const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
     if (st->d_ptab && !bf16 && st->variant == PSPIN_DENSE)
-        for (int nw : {4, 8, 12, 16}) cands.push_back({PSPIN_KERNEL_PAIRS, nw});
-    for (int bs : {64, 128, 256, 512, 1024}) cands.push_back({PSPIN_KERNEL_LISTS, bs});
-    const int old_kernel = st->kernel, old_block = st->block_size, old_nw = st->pair_nwaves;
+        for (int rows : {8, 16})
+            for (int nw : {4, 8, 12, 16}) cands.push_back({PSPIN_KERNEL_PAIRS, nw, rows});
+    for (int lb : {8, 16}) {
+        if (lb == 16 && st->variant != PSPIN_DENSE) continue;
+        for (int bs : {64, 128, 256, 512, 1024}) cands.push_back({PSPIN_KERNEL_LISTS, bs, lb});
+    }
+    const int old_kernel = st->kernel, old_block = st->block_size, old_nw = st->pair_nwaves,
+              old_rows = st->pair_rows, old_lb = st->list_batch;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     double best = 1e300;
-    Cand pick = {old_kernel, old_kernel == PSPIN_KERNEL_PAIRS ? old_nw : old_block};
+    Cand pick = {old_kernel, old_kernel == PSPIN_KERNEL_PAIRS ? old_nw : old_block,
+                 old_kernel == PSPIN_KERNEL_PAIRS ? old_rows : old_lb};
     int rc = PSPIN_OK;
     for (const Cand& c : cands) {
         if (c.kernel == PSPIN_KERNEL_PAIRS) {
-            if ((rc = build_pair_tasks(st, c.param))) break;
+            if ((rc = build_pair_tasks(st, c.param, c.rows))) break;
         } else {
             st->block_size = c.param;
+            st->list_batch = c.rows;
         }
         st->kernel = c.kernel;
         hipError_t r = ps::launch_sweep(st, 1);  // warm-up
@@ -1139,9 +1147,9 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
         }
         const double per = ms / trial_sweeps;
         if (verbose)
-            std::printf("[auto-tune] sweep N=%d nrep=%d %s %s=%d: %.4f ms/sweep\n", st->N, st->nrep,
+            std::printf("[auto-tune] sweep N=%d nrep=%d %s %s=%d depth=%d: %.4f ms/sweep\n", st->N, st->nrep,
                         c.kernel == PSPIN_KERNEL_PAIRS ? "pair-row" : "list", c.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block",
-                        c.param, per);
+                        c.param, c.rows, per);
         if (per < best) {
             best = per;
             pick = c;
@@ -1152,27 +1160,60 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
     if (rc == PSPIN_OK) {
         st->kernel = pick.kernel;
         if (pick.kernel == PSPIN_KERNEL_PAIRS) {
-            rc = build_pair_tasks(st, pick.param);
+            rc = build_pair_tasks(st, pick.param, pick.rows);
         } else {
             st->block_size = pick.param;
-            if (st->d_ptab && st->pair_nwaves != old_nw) rc = build_pair_tasks(st, old_nw);
+            st->list_batch = pick.rows;
+            if (st->d_ptab && (st->pair_nwaves != old_nw || st->pair_rows != old_rows))
+                rc = build_pair_tasks(st, old_nw, old_rows);
         }
     } else {
         st->kernel = old_kernel;
         st->block_size = old_block;
-        if (st->d_ptab) (void)build_pair_tasks(st, old_nw);
+        st->list_batch = old_lb;
+        if (st->d_ptab) (void)build_pair_tasks(st, old_nw, old_rows);
     }
     (void)hipStreamSynchronize(st->stream);
     cleanup();
     drop_graph(st);
     if (rc) return rc;
     if (verbose)
-        std::printf("[auto-tune] sweep N=%d -> %s %s=%d (%.4f ms/sweep)\n", st->N,
+        std::printf("[auto-tune] sweep N=%d -> %s %s=%d depth=%d (%.4f ms/sweep)\n", st->N,
                     pick.kernel == PSPIN_KERNEL_PAIRS ? "pair-row" : "list",
-                    pick.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block", pick.param, best);
+                    pick.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block", pick.param, pick.rows, best);
     if (kernel) *kernel = pick.kernel;
     if (block) *block = pick.kernel == PSPIN_KERNEL_PAIRS ? 64 * pick.param : pick.param;
     if (ms_per_sweep) *ms_per_sweep = best;
+    return PSPIN_OK;
+}
+
+int pspin_set_pair_tiles(pspin_state* st, int nwaves, int rows) {
+    PS_REQUIRE(st && st->d_ptab, PSPIN_ERR_STATE, "no pair-row layout (dense fully connected, 8 <= N <= 128)");
+    PS_REQUIRE(nwaves >= 1 && nwaves <= 16 && (rows == 8 || rows == 16), PSPIN_ERR_ARG,
+               "nwaves in [1, 16], rows 8 or 16");
+    drop_graph(st);
+    return build_pair_tasks(st, nwaves, rows);
+}
+
+int pspin_set_list_batch(pspin_state* st, int batch) {
+    PS_REQUIRE(st && (batch == 8 || batch == 16), PSPIN_ERR_ARG, "batch must be 8 or 16");
+    PS_REQUIRE(batch == 8 || st->variant == PSPIN_DENSE, PSPIN_ERR_ARG, "the sparse variant uses batch 8");
+    drop_graph(st);
+    st->list_batch = batch;
+    return PSPIN_OK;
+}
+
+int pspin_set_pair_schedule(pspin_state* st, int groups) {
+    PS_REQUIRE(st && groups >= 0 && groups <= st->nrep, PSPIN_ERR_ARG, "groups must be in [0, nrep]");
+    drop_graph(st);
+    st->pair_groups = groups;
+    return PSPIN_OK;
+}
+
+int pspin_get_pair_keys(const pspin_state* st, uint64_t* keys) {
+    PS_REQUIRE(st && keys, PSPIN_ERR_ARG, "null argument");
+    for (int r = 0; r < st->nrep; r++)
+        keys[r] = st->pair_groups > 0 ? ps::pair_group_key(st->cfg.seed, r % st->pair_groups) : 0;
     return PSPIN_OK;
 }
 

@@ -25,18 +25,32 @@ struct ExArgs {
     int series_cap;
 };
 
+// Accept test u < exp(delta) for delta < 0, decided exactly as in FP64: an
+// FP32 exponential (relative error well under 1e-5 over the whole range,
+// flushing to 0 below about -87) settles every u outside a 2e-5 relative
+// band around it; inside the band the FP64 exp decides.
+__device__ __forceinline__ bool accept_below(double u, double delta) {
+    const double e32 = (double)__expf((float)delta);
+    if (u < e32 * (1.0 - 2e-5)) return true;
+    if (u > e32 * (1.0 + 2e-5)) return false;
+    return u < exp(delta);
+}
+
 __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
-    // LDS: energies and betas (read once, coalesced), then the permutation.
+    // LDS: the permutation and the energies by slot (sEp[s] = E[perm[s]],
+    // swapped together), betas, per-pair accept counts.  Lane r owns ladder
+    // column r at every level, so levels need no barrier.
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
     const int NT = a.NT, nr = a.nrep, n = NT * nr;
-    double* sE = lds;
+    double* sEp = lds;
     double* sB = lds + n;
     long long* sacc = (long long*)(lds + n + NT);  // [NT-1] accepted swaps this round
     int* sperm = (int*)(sacc + NT);
     for (int i = lane; i < n; i += 64) {
-        sE[i] = a.E[i];
-        sperm[i] = a.perm[i];
+        const int p = a.perm[i];
+        sperm[i] = p;
+        sEp[i] = a.E[p];
     }
     for (int i = lane; i < NT; i += 64) sB[i] = a.beta_sched[i];
     // Jump table: k steps of the 48-bit LCG, k = 0..64, as one multiply-add.
@@ -52,25 +66,31 @@ __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
         for (int c0 = 0; c0 < nr; c0 += 64) {
             const int r = c0 + lane;
             const bool valid = r < nr;
-            double delta = 0.0;
-            if (valid) delta = db * (sE[sperm[t * nr + r]] - sE[sperm[(t + 1) * nr + r]]);
+            const int sa = t * nr + r, sb = sa + nr;
+            double ea = 0.0, eb = 0.0, delta = 0.0;
+            if (valid) {
+                ea = sEp[sa];
+                eb = sEp[sb];
+                delta = db * (ea - eb);
+            }
             const bool need = valid && !(delta >= 0.0);
             const unsigned long long m = __ballot(need);
             bool accept = valid;
             if (need) {
                 const int k = __popcll(m & lt) + 1;
                 const uint64_t x = (jA[k] * X + jC[k]) & kMask48;
-                accept = lcg48_unit(x) < exp(delta);
+                accept = accept_below(lcg48_unit(x), delta);
             }
             const int tot = __popcll(m);
             X = (jA[tot] * X + jC[tot]) & kMask48;
             if (valid && accept) {
-                const int tmp = sperm[t * nr + r];
-                sperm[t * nr + r] = sperm[(t + 1) * nr + r];
-                sperm[(t + 1) * nr + r] = tmp;
+                const int tmp = sperm[sa];
+                sperm[sa] = sperm[sb];
+                sperm[sb] = tmp;
+                sEp[sa] = eb;
+                sEp[sb] = ea;
             }
             nacc += __popcll(__ballot(valid && accept));
-            __syncthreads();
         }
         if (lane == 0) sacc[t] = nacc;
     }
@@ -89,7 +109,7 @@ __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
     if (a.series) {
         const long long round = *a.series_n;
         double* row = a.series + (size_t)(round % a.series_cap) * nr;
-        for (int r = lane; r < nr; r += 64) row[r] = sE[sperm[(NT - 1) * nr + r]];
+        for (int r = lane; r < nr; r += 64) row[r] = sEp[(NT - 1) * nr + r];
     }
     __syncthreads();
     if (lane == 0) {

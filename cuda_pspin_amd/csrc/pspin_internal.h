@@ -27,6 +27,9 @@ constexpr uint32_t kTagG4 = 4;
 // spin initialisation, so no sweep draw reuses an initialisation block.
 constexpr uint32_t kCtrStep0 = 16;
 constexpr uint32_t kCtrStep1 = 17;
+constexpr uint32_t kCtrPair = 18;    // pair draws of a shared pair schedule (group key)
+constexpr uint64_t kPairKeyOffset = 7000;  // group g's key: splitmix64 of (seed + 7000 + g)
+
 
 // ---------------------------------------------------------------------------
 // splitmix64 (Steele, Lea, Flood 2014) -- the reference's seed chain
@@ -37,6 +40,12 @@ PS_HD uint64_t splitmix64(uint64_t* s) {
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
     return z ^ (z >> 31);
+}
+
+// Philox key of pair-schedule group g (replicas r with r % groups == g).
+PS_HD uint64_t pair_group_key(uint64_t seed, int g) {
+    uint64_t s = seed + kPairKeyOffset + (uint64_t)g;
+    return splitmix64(&s);
 }
 
 // ---------------------------------------------------------------------------
@@ -195,6 +204,8 @@ struct pspin_state {
     int4* d_tasks = nullptr;
     int* d_wave_tasks = nullptr;
     int pair_nwaves = 0;
+    int pair_rows = 8;  // rows per task of the pair-row kernel (8 or 16)
+    int list_batch = 8; // entries in flight per thread in the list kernel (8 or 16; dense only)
 
     // replicas
     double* d_spins = nullptr;  // [nrep][N][2]
@@ -232,6 +243,7 @@ struct pspin_state {
     int log_n = 0;
 
     // kernel choice and profiling
+    int pair_groups = 0;  // 0: every replica draws its own pairs; G: replicas r = g mod G share
     int kernel = PSPIN_KERNEL_LISTS;
     int block_size = 0;
     bool prof = false;

@@ -45,6 +45,8 @@ namespace ps {
 
 struct SweepArgs {
     int N, nrep, h2_active, n_sweeps;
+    int pair_groups;        // 0: pairs from the replica stream; G > 0: from group (rep % G)'s stream
+    uint64_t pair_seed;     // run seed the group keys derive from
     double* spins;
     const float* g2;
     const int* site_off;
@@ -83,11 +85,20 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 // Draws and trigonometry for one step (pair pick src/mc.cu:129-141; angles
 // include/spins.h:25-32 or include/spins_sparse.h:34-41).
+// With a shared pair schedule (gkey != 0) the two pair uniforms come from
+// the group's stream; everything else stays per replica.
 template <bool CUBE>
-__device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t sweep, int step, int N, double beta) {
+__device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t gkey, uint64_t sweep, int step, int N,
+                                             double beta) {
     uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep0}, k0, k1);
     U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep1}, k0, k1);
+    if (gkey) {
+        const U4 z = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrPair},
+                                   (uint32_t)gkey, (uint32_t)(gkey >> 32));
+        x.x = z.x;
+        x.y = z.y;
+    }
     StepPre p;
     double u1 = u32_to_unit(x.x), u2 = u32_to_unit(x.y);
     int ri = (int)(u1 * N);
@@ -112,10 +123,9 @@ __device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t sweep, int 
 }
 
 // Local-field contribution of the thread for the current step, LISTS layout.
-// The two site lists are walked as one range; each thread issues kBatch
-// independent 8-byte loads before using any, so enough bytes are in flight
-// to cover L2/MALL latency.  Entries past the end read as g = 0.
-constexpr int kListBatch = 8;
+// The two site lists are walked as one range; each thread issues LB (8 or
+// 16) independent 8-byte loads before using any, so enough bytes are in
+// flight to cover L2/MALL latency.  Entries past the end read as g = 0.
 
 // Coupling word of a list entry: FP32, or a BF16 hi/lo pair (hi in the low
 // half) whose sum carries ~16 mantissa bits; accumulation stays FP32.
@@ -125,7 +135,7 @@ __device__ __forceinline__ float coupling(uint32_t w) {
     return __uint_as_float(w << 16) + __uint_as_float(w & 0xffff0000u);
 }
 
-template <bool BF16>
+template <bool BF16, int kListBatch>
 __device__ __forceinline__ float field_lists(const SweepArgs& a, const float2* s32, int i0, int j0, float2 di,
                                              float2 dj, int tid, int bdim) {
     float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
@@ -159,8 +169,9 @@ __device__ __forceinline__ float field_lists(const SweepArgs& a, const float2* s
 // Local-field contribution of the thread for the current step, PAIRS layout
 // (fully connected, channel 0: f = a_p0 a_p1 conj(a_p2) conj(a_p3)).
 // For site x and others a<b<c: partner is a when o(b) > x, else c.
-// Rows are consumed kPairBatch at a time with all loads issued first.
-constexpr int kPairBatch = 8;
+// Rows are consumed PB at a time with all loads issued first (PB = the
+// rows per task of the work list, 8 or 16: one memory round trip per task).
+template <int PB>
 __device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s32, const uint32_t* pbc, int i0,
                                              int j0, float2 di, float2 dj, int w, int lane) {
     float local = 0.f;
@@ -190,16 +201,16 @@ __device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[j] = make_float2(0.f, 0.f);
         const float* base = a.ptab + (size_t)site * a.slice_len + P0;
-        for (int r = tk.z; r < tk.w; r += kPairBatch) {
-            float4 g[kPairBatch];
+        for (int r = tk.z; r < tk.w; r += PB) {
+            float4 g[PB];
 #pragma unroll
-            for (int u = 0; u < kPairBatch; u++) {
+            for (int u = 0; u < PB; u++) {
                 const int ar = r + u;
                 g[u] = (ar < tk.w && P0 < a.row_len[ar]) ? *(const float4*)(base + a.row_off[ar])
                                                          : make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
-            for (int u = 0; u < kPairBatch; u++) {
+            for (int u = 0; u < PB; u++) {
                 const int ar = min(r + u, tk.w - 1);
                 const float2 ta = s32[ar + (ar >= site)];
                 acc[0].x += g[u].x * ta.x; acc[0].y += g[u].x * ta.y;
@@ -220,7 +231,7 @@ __device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s
     return local;
 }
 
-template <bool CUBE, bool PAIRS, bool BF16>
+template <bool CUBE, bool PAIRS, bool BF16, int PB = 8>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     __shared__ double2 s64[kMaxN];
     __shared__ float2 s32[kMaxN];
@@ -249,13 +260,14 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     const double beta = a.betas[rep];
     const uint64_t seed = a.seeds[rep];
     const uint64_t sweep0 = a.ctr[rep];
+    const uint64_t gkey = a.pair_groups > 0 ? pair_group_key(a.pair_seed, rep % a.pair_groups) : 0;
     double E = 0.0;
     long long nacc = 0, nprop = 0;
     if (tid == 0) E = a.E[rep];
 
     for (int sw = 0; sw < a.n_sweeps; sw++) {
         const uint64_t sweep = sweep0 + (uint64_t)sw;
-        for (int k = tid; k < nsteps; k += bdim) pre[k] = make_step<CUBE>(seed, sweep, k, N, beta);
+        for (int k = tid; k < nsteps; k += bdim) pre[k] = make_step<CUBE>(seed, gkey, sweep, k, N, beta);
         __syncthreads();
         StepRec* logp = nullptr;
         if (a.log && a.log_base + sw < a.log_cap)
@@ -332,9 +344,9 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 local = -(di.x * Fi.x - di.y * Fi.y) - (dj.x * Fj.x - dj.y * Fj.y);
             }
             if (PAIRS)
-                local += field_pairs(a, s32, pbc, i0, j0, di, dj, w, lane);
+                local += field_pairs<PB>(a, s32, pbc, i0, j0, di, dj, w, lane);
             else
-                local += field_lists<BF16>(a, s32, i0, j0, di, dj, tid, bdim);
+                local += field_lists<BF16, PB>(a, s32, i0, j0, di, dj, tid, bdim);
             local = wave_sum(local);
             if (lane == 0) part[step & 1][w] = local;
             __syncthreads();
@@ -382,6 +394,8 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
     a.npairs = st->pair_npairs;
     a.tasks = st->d_tasks;
     a.wave_tasks = st->d_wave_tasks;
+    a.pair_groups = st->pair_groups;
+    a.pair_seed = st->cfg.seed;
     const bool pairs = st->kernel == PSPIN_KERNEL_PAIRS;
     int bs = pairs ? 64 * st->pair_nwaves : st->block_size;
     if (bs <= 0) {
@@ -390,11 +404,18 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
         bs = per_step >= 8192 ? 1024 : per_step >= 1024 ? 256 : 64;
     }
     const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
+    const bool lb16 = st->list_batch == 16;
     if (st->variant == PSPIN_SPARSE) {
         if (bf16) hipLaunchKernelGGL((k_sweep<true, false, true>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
         else hipLaunchKernelGGL((k_sweep<true, false, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
     } else if (pairs) {
-        hipLaunchKernelGGL((k_sweep<false, true, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        if (st->pair_rows == 16)
+            hipLaunchKernelGGL((k_sweep<false, true, false, 16>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        else
+            hipLaunchKernelGGL((k_sweep<false, true, false, 8>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+    } else if (lb16) {
+        if (bf16) hipLaunchKernelGGL((k_sweep<false, false, true, 16>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        else hipLaunchKernelGGL((k_sweep<false, false, false, 16>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
     } else {
         if (bf16) hipLaunchKernelGGL((k_sweep<false, false, true>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
         else hipLaunchKernelGGL((k_sweep<false, false, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);

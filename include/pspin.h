@@ -230,6 +230,23 @@ int pspin_device_info(int dev, size_t* free_bytes, size_t* total_bytes, char* na
 #define PSPIN_COUPLING_F32 0
 #define PSPIN_COUPLING_BF16X2 1 /* BF16 hi/lo split, FP32 accumulation       */
 int pspin_set_kernel(pspin_state* st, int kernel, int block_size);
+/* Pair schedule.  groups = 0 (default): every replica draws its own pair,
+ * as the reference does (src/mc.cu:129-141, per-replica curand states).
+ * groups = G > 0: replicas r with equal r mod G draw their pairs from one
+ * shared stream (key splitmix64 of seed + 7000 + g); angles and acceptance
+ * stay per replica.  Each replica's transition kernel is unchanged (the pair
+ * sequence does not depend on the state), so every stationary law is too.
+ * With G = 8 the replicas of one group run on one XCD and read the same
+ * coupling slices at each step from that XCD's L2.  pspin_get_pair_keys
+ * gives each replica's group key (0 with per-replica pairs). */
+int pspin_set_pair_schedule(pspin_state* st, int groups);
+/* Tile shape of the pair-row kernel: waves per workgroup (1..16) and rows per
+ * task (8 or 16, the loads in flight per lane).  pspin_autotune picks it. */
+int pspin_set_pair_tiles(pspin_state* st, int nwaves, int rows);
+/* Coupling entries in flight per thread in the list kernel (8 or 16; 16 for
+ * the dense variant only).  pspin_autotune picks it. */
+int pspin_set_list_batch(pspin_state* st, int batch);
+int pspin_get_pair_keys(const pspin_state* st, uint64_t* keys);
 /* Timing autotuner for the sweep (replaces optimal_block_size_for_sweep,
  * src/mc.cu:709-744, which only queries occupancy): times each candidate
  * kernel / tile shape for trial_sweeps sweeps on the live state, restores the

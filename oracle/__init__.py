@@ -82,6 +82,8 @@ def lib() -> C.CDLL:
             "or_u64_to_unit": (C.c_double, [C.c_uint32, C.c_uint32]),
             "or_normal": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint64]),
             "or_step_uniforms": (None, [C.c_uint64, C.c_uint64, C.c_int, _dp]),
+            "or_step_uniforms_grouped": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, _dp]),
+            "or_pair_group_key": (C.c_uint64, [C.c_uint64, C.c_int]),
             "or_cpu_baseline_sweeps": (C.c_double, [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _u8p, _dp, _u64p,
                                                     C.c_uint64, C.c_int, _dp, _i64p, C.c_int]),
             "or_max_threads": (C.c_int, []),
@@ -283,6 +285,16 @@ def philox(ctr, key) -> np.ndarray:
 
 def normal(seed: int, tag: int, k: int) -> float:
     return lib().or_normal(seed, tag, k)
+
+
+def step_uniforms_grouped(rseed: int, gkey: int, sweep: int, step: int) -> np.ndarray:
+    u = np.zeros(6)
+    lib().or_step_uniforms_grouped(rseed, gkey, sweep, step, u)
+    return u
+
+
+def pair_group_key(seed: int, g: int) -> int:
+    return int(lib().or_pair_group_key(seed, g))
 
 
 def step_uniforms(rseed: int, sweep: int, step: int) -> np.ndarray:

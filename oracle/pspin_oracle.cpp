@@ -542,6 +542,26 @@ void or_step_uniforms(uint64_t rseed, uint64_t sweep, int step, double* u) {
     u[5] = or_u32_to_unit(y[1]);
 }
 
+// Shared pair schedule (this build's option): the two pair uniforms of the
+// step come from the group stream keyed by gkey, counter word 18; the rest
+// as or_step_uniforms.  gkey = 0 means per-replica pairs.
+void or_step_uniforms_grouped(uint64_t rseed, uint64_t gkey, uint64_t sweep, int step, double* u) {
+    or_step_uniforms(rseed, sweep, step, u);
+    if (gkey) {
+        uint32_t key[2] = {(uint32_t)gkey, (uint32_t)(gkey >> 32)};
+        uint32_t c[4] = {(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, 18u}, z[4];
+        or_philox4x32_10(c, key, z);
+        u[0] = or_u32_to_unit(z[0]);
+        u[1] = or_u32_to_unit(z[1]);
+    }
+}
+
+// Group key of pair-schedule group g: splitmix64 of (seed + 7000 + g).
+uint64_t or_pair_group_key(uint64_t seed, int g) {
+    uint64_t s = seed + 7000ULL + (uint64_t)g;
+    return or_splitmix64(&s);
+}
+
 // ---------------------------------------------------------------------------
 // CPU baseline: OpenMP transcription of the reference dense sweep
 // (src/mc.cu:66-318): one replica chain per thread, FP64 throughout, the

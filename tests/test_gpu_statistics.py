@@ -23,9 +23,10 @@ N, NT, NR = 10, 5, 2
 BURN, MEAS = 500, 2000
 
 
-def _gpu_run(pspin, seed, beta):
+def _gpu_run(pspin, seed, beta, groups):
     cfg = pspin.SimConfig.default(N=N, nrep=NT * NR, seed=seed, fmc_mode=0)
     st = pspin.PSpin(cfg, pspin.DENSE)
+    st.set_pair_schedule(groups)
     st.pt_init(NT, NR, beta, seed + 5000)
     s0 = st.spins()
     g2 = st.g2()
@@ -64,12 +65,15 @@ def _cpu_run(oracle, seed, beta, d):
     return np.concatenate([e, q2, [acc], out["ex_acc"] / out["ex_prop"]])
 
 
-def test_gpu_pt_matches_reference_logic_statistically(pspin, oracle):
+@pytest.mark.parametrize("groups", [0, NR])
+def test_gpu_pt_matches_reference_logic_statistically(pspin, oracle, groups):
+    """groups = NR: each ladder's replicas share one pair stream (the layout
+    the benchmark uses), the ladders keep independent streams."""
     T, beta = oracle.temperatures(NT, 0.5, 2.0, 1)
     seeds = [1000 + 7 * s for s in range(12)]
     gpu, data = [], []
     for sd in seeds:
-        o, d = _gpu_run(pspin, sd, beta)
+        o, d = _gpu_run(pspin, sd, beta, groups)
         gpu.append(o)
         data.append(d)
     with cf.ThreadPoolExecutor(max_workers=8) as ex:

@@ -52,6 +52,7 @@ def replay_check(O, st, n_sweeps, betas):
     s0 = st.spins()
     E0, _, _ = st.results()
     seeds, sweep0 = st.rng_info()
+    keys = st.pair_keys()
     st.set_betas(betas)
     st.enable_step_log(n_sweeps)
     st.sweep(n_sweeps)
@@ -70,7 +71,7 @@ def replay_check(O, st, n_sweeps, betas):
         for sw in range(n_sweeps):
             for k in range(steps):
                 rec = log[r, sw, k]
-                u = O.step_uniforms(int(seeds[r]), sweep0 + sw, k)
+                u = O.step_uniforms_grouped(int(seeds[r]), int(keys[r]), sweep0 + sw, k)
                 i0, j0 = O.pair_pick(N, u[0], u[1])
                 assert (int(rec["i0"]), int(rec["j0"])) == (i0, j0), (r, sw, k)
                 if st.variant == 0:
@@ -134,6 +135,42 @@ def test_step_replay(pspin, oracle, name, variant, kernel, kw, T, nsw, fmt):
     print(f"  max |dE32-dE64|/S = {stats['worst_scaled']:.2e}  vs |dE64|: {stats['worst_rel']:.2e}  "
           f"max |dE-dE64| = {stats['max_abs']:.2e}  rms = {stats['rms_abs']:.2e}  acc={stats['acc_rate']:.3f}",
           flush=True)
+    st.close()
+
+
+@pytest.mark.parametrize("name,kernel,kw,groups,tiles", [
+    ("dense FC N=64 pairs, 8 pair groups (bench system)", 2, dict(N=64, fmc_mode=0), 8, None),
+    ("dense FC N=64 pairs, 8 pair groups, 12 waves x 16 rows", 2, dict(N=64, fmc_mode=0), 8, (12, 16)),
+    ("dense FC N=40 pairs, per-replica pairs, 4 waves x 16 rows", 2, dict(N=40, fmc_mode=0), 0, (4, 16)),
+    ("dense uniform N=32 lists, 4 pair groups", 1, dict(N=32, fmc_mode=2, gamma=0.05), 4, None),
+    ("dense uniform N=64 lists, 16 entries in flight", 1, dict(N=64, fmc_mode=2, gamma=0.05), 0, 16),
+])
+def test_step_replay_shared_pair_schedule(pspin, oracle, name, kernel, kw, groups, tiles):
+    """Replicas r = g (mod G) draw their pairs from group g's stream; the rest
+    of each step stays per replica.  Replayed step by step like the default."""
+    print(f"[step replay] {name}", flush=True)
+    cfg = pspin.SimConfig.default(nrep=16, seed=4321, **kw)
+    st = pspin.PSpin(cfg, 0)
+    st.set_kernel(kernel, 0)
+    if isinstance(tiles, tuple):
+        st.set_pair_tiles(*tiles)
+    elif tiles:
+        st.set_list_batch(tiles)
+    st.set_pair_schedule(groups)
+    keys = st.pair_keys()
+    if groups:
+        assert len(set(int(k) for k in keys)) == groups
+        for r in range(16):
+            assert int(keys[r]) == oracle.pair_group_key(cfg.seed, r % groups)
+    else:
+        assert not np.any(keys)
+    betas = np.linspace(0.5, 3.0, 16)
+    stats = replay_check(oracle, st, 2, betas)
+    log, _ = st.step_log()
+    if groups:  # same group, same pairs; different groups, different pair sequences
+        assert np.array_equal(log[0]["i0"], log[groups]["i0"]) and np.array_equal(log[0]["j0"], log[groups]["j0"])
+        assert not np.array_equal(log[0]["i0"], log[1]["i0"])
+    print(f"  max |dE32-dE64|/S = {stats['worst_scaled']:.2e}  acc={stats['acc_rate']:.3f}", flush=True)
     st.close()
 
 
