@@ -72,7 +72,7 @@ void free_all(pspin_state* st) {
     st->graph = nullptr;
     dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48);
     dfree(st->d_series); dfree(st->d_series_n); dfree(st->d_log);
-    dfree(st->d_ptab); dfree(st->d_pair_bc);
+    dfree(st->d_ptab); dfree(st->d_pinfo);
     dfree(st->d_tasks); dfree(st->d_wave_tasks);
     for (auto& pr : st->ev_sweep) { st->ev_free.push_back(pr.first); st->ev_free.push_back(pr.second); }
     for (auto& pr : st->ev_exch) { st->ev_free.push_back(pr.first); st->ev_free.push_back(pr.second); }
@@ -135,34 +135,60 @@ int build_site_lists(pspin_state* st, const std::vector<Term>& terms) {
     return PSPIN_OK;
 }
 
-// Work list of the pair-row kernel for `nw` waves: blocks of 64 chunks of 4
-// pairs, split into ranges of 8 rows a < b_max(block) for both sites of the
-// step, dealt longest first to the least loaded wave.
+// Work list of the pair-row kernel for at most `nw_req` waves.  The tiles
+// (slot 0 = i0, slot 1 = j0) x (pair block k) are laid end to end and cut into
+// contiguous runs, one per wave, at multiples of 4 rows; a run that crosses a
+// tile boundary becomes several segments.  The largest run cost is minimised
+// by bisection over the cost bound with a greedy fill, where a segment costs
+// kSegCost plus kRowCost per row (instruction counts of the kernel's set-up
+// and row loop).  `rows` is the kernel's rows in flight per batch (8 or 16).
 int build_pair_tasks(pspin_state* st, int nw_req, int rows) {
-    const std::vector<uint32_t>& pbc = st->h_pair_bc;
-    const int npairs = (int)pbc.size();
-    const int nchunks = (npairs + 3) / 4, nblocks = (nchunks + 63) / 64;
-    const int kRows = rows == 16 ? 16 : 8;
-    std::vector<std::array<int, 4>> tasks;
-    for (int slot = 0; slot < 2; slot++)
-        for (int blk = 0; blk < nblocks; blk++) {
-            const int bmax = (int)(pbc[(size_t)blk * 256] & 0xffff);  // rows a < bmax
-            for (int a0 = 0; a0 < bmax; a0 += kRows) tasks.push_back({slot, blk * 64, a0, std::min(bmax, a0 + kRows)});
-        }
-    const int nw = std::max(1, std::min<int>(nw_req, (int)tasks.size()));
-    std::stable_sort(tasks.begin(), tasks.end(),
-                     [](const std::array<int, 4>& p, const std::array<int, 4>& q) { return p[3] - p[2] > q[3] - q[2]; });
-    std::vector<std::vector<std::array<int, 4>>> per(nw);
-    std::vector<int> load(nw, 0);
-    for (const auto& t : tasks) {
-        int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        per[w].push_back(t);
-        load[w] += (t[3] - t[2]) + 2;  // rows plus per-task set-up
+    constexpr long long kSegCost = 64, kRowCost = 9;
+    const int nblk = (int)st->h_tile_rows.size();
+    struct Seg { int slot, k, y0, nr; };
+    auto fill = [&](long long C, std::vector<std::vector<Seg>>* out) -> int {
+        int waves = 1;
+        long long c = 0;
+        if (out) out->assign(1, {});
+        for (int slot = 0; slot < 2; slot++)
+            for (int k = 0; k < nblk; k++) {
+                int y = 0;
+                const int ny = st->h_tile_rows[k];
+                while (y < ny) {
+                    long long avail = C - c - kSegCost;
+                    int take = avail >= 4 * kRowCost ? (int)(avail / kRowCost) / 4 * 4 : 0;
+                    take = std::min(take, ny - y);
+                    if (take == 0) {
+                        if (c == 0) return 1 << 30;  // bound below one segment
+                        waves++;
+                        c = 0;
+                        if (out) out->push_back({});
+                        continue;
+                    }
+                    if (out) out->back().push_back({slot, k, y, take});
+                    c += kSegCost + kRowCost * take;
+                    y += take;
+                }
+            }
+        return waves;
+    };
+    long long total = 0;
+    for (int k = 0; k < nblk; k++) total += 2 * (kSegCost + kRowCost * st->h_tile_rows[k]);
+    const int nwr = std::max(1, std::min(nw_req, 16));
+    long long lo = kSegCost + 4 * kRowCost, hi = total;
+    while (lo < hi) {
+        const long long mid = (lo + hi) / 2;
+        if (fill(mid, nullptr) <= nwr) hi = mid;
+        else lo = mid + 1;
     }
+    std::vector<std::vector<Seg>> per;
+    fill(lo, &per);
+    const int nw = (int)per.size();
     std::vector<int4> flat;
     std::vector<int> wt(nw + 1, 0);
     for (int w = 0; w < nw; w++) {
-        for (const auto& t : per[w]) flat.push_back(make_int4(t[0], t[1], t[2], t[3]));
+        for (const Seg& g : per[w])
+            flat.push_back(make_int4(g.slot | (g.nr << 16), 256 * g.k, g.y0, st->h_tile_off[g.k] + 256 * g.y0));
         wt[w + 1] = (int)flat.size();
     }
     dfree(st->d_tasks);
@@ -172,58 +198,65 @@ int build_pair_tasks(pspin_state* st, int nw_req, int rows) {
     PS_TRY_HIP(cpy(st, st->d_tasks, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
     PS_TRY_HIP(cpy(st, st->d_wave_tasks, wt.data(), wt.size() * sizeof(int), hipMemcpyHostToDevice));
     st->pair_nwaves = nw;
-    st->pair_rows = kRows;
+    st->pair_rows = rows == 16 ? 16 : 8;
     return PSPIN_OK;
 }
 
-// Dense per-site pair-row slices for the fully connected case (every quartet
-// channel 0).  Others of site x are relabelled 0..M-1 (M = N-1, o(y) = y + (y >= x)).
-// Pairs (b<c) are ordered by b descending, so the pairs with b > a -- row a --
-// are a prefix of length C(M-1-a, 2).  Rows are padded to a multiple of 4
-// floats with zeros.  The task list splits each 64-lane block of 4-pair chunks
-// into row ranges of at most kRows rows and deals them longest-first to the
-// waves of the workgroup.
+// Dense per-site pair tiles for the fully connected case (every quartet
+// channel 0).  The other sites of x are relabelled 0..M-1 (M = N-1,
+// o(y) = y + (y >= x)) and each unordered triple a < b < c of them is stored
+// once, at (row o(a), pair (b,c)).  Pairs (b<c) are ordered by b descending
+// and cut into blocks of 256 (64 lanes x 4); block k holds rows
+// y = 0 .. b_max(k) (a < b <= b_max means o(a) <= b_max) rounded up to 4, as a
+// dense [rows][256] FP32 tile with zeros where a >= b, at y = x and past the
+// last pair.  Rows are indexed by the site itself so the kernel reads the row
+// amplitude s[y] at a fixed offset.  pinfo[x][P] = 8 o(b) | 8 o(c) << 16 |
+// (o(b) > x) << 31: the LDS byte offsets of the pair's amplitudes and the
+// partner flag (partner a when o(b) > x, else c).
 int build_pair_layout(pspin_state* st, const std::vector<double>& g4, const std::vector<uint8_t>& mask) {
     const int N = st->N, M = N - 1;
-    const int nrows = M - 2;
     std::vector<uint32_t> pbc;
     for (int b = M - 2; b >= 1; b--)
         for (int c = b + 1; c < M; c++) pbc.push_back((uint32_t)b | ((uint32_t)c << 16));
-    const int npairs = (int)pbc.size();
-    std::vector<int> roff(nrows), rlen(nrows);
+    const int npairs = (int)pbc.size(), nblk = (npairs + 255) / 256, npad = 256 * nblk;
+    const int n4 = (N + 3) / 4 * 4;
+    std::vector<int> trows(nblk), toff(nblk);
     long long off = 0;
-    for (int a = 0; a < nrows; a++) {
-        long long L = (long long)(M - 1 - a) * (M - 2 - a) / 2;
-        roff[a] = (int)off;
-        rlen[a] = (int)((L + 3) / 4 * 4);
-        off += rlen[a];
+    for (int k = 0; k < nblk; k++) {
+        const int bmax = (int)(pbc[(size_t)256 * k] & 0xffff);
+        trows[k] = std::min(n4, (bmax + 1 + 3) / 4 * 4);
+        toff[k] = (int)off;
+        off += 256LL * trows[k];
     }
-    const int slice = (int)((off + 15) / 16 * 16);
+    const long long slice = off;
     std::vector<float> tab((size_t)N * slice, 0.0f);
+    std::vector<uint32_t> info((size_t)N * npad, 0u);
     for (int x = 0; x < N; x++) {
         float* sl = &tab[(size_t)x * slice];
-        for (int a = 0; a < nrows; a++) {
-            const long long L = (long long)(M - 1 - a) * (M - 2 - a) / 2;
-            for (long long P = 0; P < L; P++) {
-                const int b = pbc[P] & 0xffff, c = pbc[P] >> 16;
-                int v[4] = {x, a + (a >= x), b + (b >= x), c + (c >= x)};
+        for (int P = 0; P < npairs; P++) {
+            const int b = pbc[P] & 0xffff, c = pbc[P] >> 16;
+            const int ob = b + (b >= x), oc = c + (c >= x), k = P / 256, p = P % 256;
+            info[(size_t)x * npad + P] = (uint32_t)(8 * ob) | ((uint32_t)(8 * oc) << 16) | ((uint32_t)(b >= x) << 31);
+            for (int a = 0; a < b; a++) {
+                const int y = a + (a >= x);
+                int v[4] = {x, y, ob, oc};
                 std::sort(v, v + 4);
                 const long long q = (long long)v[3] * (v[3] - 1) * (v[3] - 2) * (v[3] - 3) / 24 +
                                     (long long)v[2] * (v[2] - 1) * (v[2] - 2) / 6 + (long long)v[1] * (v[1] - 1) / 2 + v[0];
-                sl[roff[a] + P] = (mask[q] == 1u) ? (float)g4[q] : 0.0f;
+                sl[toff[k] + 256LL * y + p] = (mask[q] == 1u) ? (float)g4[q] : 0.0f;
             }
         }
     }
     st->pair_slice_len = slice;
     st->pair_npairs = npairs;
+    st->pair_npad = npad;
+    st->h_tile_rows = trows;
+    st->h_tile_off = toff;
     PS_TRY_HIP(dmalloc(&st->d_ptab, tab.size()));
-    PS_TRY_HIP(dmalloc(&st->d_pair_bc, pbc.size()));
+    PS_TRY_HIP(dmalloc(&st->d_pinfo, info.size()));
     PS_TRY_HIP(cpy(st, st->d_ptab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
-    st->h_row_off = roff;
-    st->h_row_len = rlen;
-    st->h_pair_bc = pbc;
-    PS_TRY_HIP(cpy(st, st->d_pair_bc, pbc.data(), pbc.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    return build_pair_tasks(st, 16, 8);
+    PS_TRY_HIP(cpy(st, st->d_pinfo, info.data(), info.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return build_pair_tasks(st, 16, 16);
 }
 
 // splitmix64 frequencies (src/mc.cu:449-460).

@@ -193,18 +193,17 @@ struct pspin_state {
     std::vector<uint2> h_site_ent;  // FP32 entries, kept to re-encode the coupling words
     int coupling_format = PSPIN_COUPLING_F32;
 
-    // dense fully connected per-site pair-row slices (PSPIN_KERNEL_PAIRS)
-    float* d_ptab = nullptr;      // [N][pair_slice_len]
-    int pair_slice_len = 0;
-    std::vector<int> h_row_off;   // [N-3]
-    std::vector<int> h_row_len;   // [N-3]
-    uint32_t* d_pair_bc = nullptr;
-    std::vector<uint32_t> h_pair_bc;
-    int pair_npairs = 0;
+    // dense fully connected per-site pair tiles (PSPIN_KERNEL_PAIRS)
+    float* d_ptab = nullptr;       // [N][pair_slice_len]
+    uint32_t* d_pinfo = nullptr;   // [N][pair_npad] 8 o(b) | 8 o(c) << 16 | partner-a flag << 31
+    long long pair_slice_len = 0;  // floats per site
+    int pair_npairs = 0, pair_npad = 0;
+    std::vector<int> h_tile_rows;  // rows of pair block k's tile (multiple of 4)
+    std::vector<int> h_tile_off;   // offset of pair block k's tile in a slice (floats)
     int4* d_tasks = nullptr;
     int* d_wave_tasks = nullptr;
     int pair_nwaves = 0;
-    int pair_rows = 8;  // rows per task of the pair-row kernel (8 or 16)
+    int pair_rows = 16; // rows in flight per batch in the pair-row kernel (8 or 16)
     int list_batch = 8; // entries in flight per thread in the list kernel (8 or 16; dense only)
 
     // replicas

@@ -23,16 +23,19 @@
 //  LISTS  site x owns a contiguous list of (p | m<<8 | n<<16, g) entries
 //         (any channel pattern: FMC, comb, sparse quartets); coalesced 8-byte
 //         loads, the three amplitudes from LDS.
-//  PAIRS  fully connected (every quartet channel 0): site x owns a dense
-//         FP32 slice indexed by (row a, pair (b,c)) over the other sites
-//         relabelled 0..N-2, rows a < b < c.  Pairs are ordered by b
-//         descending so row a is a prefix; a lane owns 4 consecutive pairs
-//         and walks rows with one 16-byte load per row, accumulating
-//         W_bc = sum_a g_abc t_a against the broadcast amplitude t_a.  The
-//         local field is then sum_bc U_bc W_bc (partner a) or U_bc conj(W_bc)
-//         (partner c) with U_bc a product of two amplitudes: 4 bytes and 2 FMA
-//         per coupling, no index data.  A static task list (chunk block x
-//         row range, longest-first to waves) balances the triangular rows.
+//  PAIRS  fully connected (every quartet channel 0): site x owns dense FP32
+//         tiles indexed by (row y, pair (b,c)) over the other sites, each
+//         triple a < b < c of them once, at row y = the site of a.  Pairs are
+//         ordered by b descending and cut into blocks of 256; block k's tile
+//         holds rows 0..b_max(k) with zeros where a >= b.  A lane owns 4
+//         consecutive pairs and walks rows with one 16-byte load per row,
+//         accumulating W_bc = sum_y g_ybc s~_y against the broadcast row
+//         amplitude s~_y (two rows per 16-byte LDS read).  No predicate or
+//         index arithmetic per row: 4 bytes and 2 FMA per coupling.  The local
+//         field is then sum_bc U_bc W_bc (partner a) or U_bc conj(W_bc)
+//         (partner c) with U_bc a product of two amplitudes, from a per-site
+//         table of LDS offsets and partner flags.  A static list of row
+//         segments balances the waves.
 //
 // Randomness: Philox4x32-10 keyed by the replica seed (the reference's
 // splitmix64 chain, src/mc.cu:418-423) and countered by (sweep, step), all N/2
@@ -61,26 +64,30 @@ struct SweepArgs {
     int log_cap;   // sweeps per replica in the log
     int log_base;  // sweeps already logged
     // PAIRS layout
-    const float* ptab;        // [N][slice_len]
-    int slice_len;
-    // Row tables travel in the kernel arguments (constant memory), so the
-    // per-row reads are scalar loads with no aliasing doubt.
-    int row_off[kMaxPairsN - 3];  // offset of row a in a slice (floats)
-    int row_len[kMaxPairsN - 3];  // length of row a, multiple of 4
-    const uint32_t* pair_bc;  // [npairs] b | c << 16
-    int npairs;
-    const int4* tasks;        // {slot (0: i0, 1: j0), first chunk, a0, a1}
-    const int* wave_tasks;    // [nwaves+1] task ranges per wave
+    const float* ptab;        // [N][slice_len] per-site pair tiles
+    const uint32_t* pinfo;    // [N][npad] per-site pair amplitude offsets and partner flags
+    long long slice_len;
+    int npad;
+    const int4* tasks;        // {slot | rows << 16, first pair, first row y0, tile offset of row y0}
+    const int* wave_tasks;    // [nwaves+1] segment ranges per wave
 };
 
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Sum over the 64 lanes of a wave with DPP moves (no LDS round trips): pairs
+// and quads by quad_perm, rows of 16 by row_ror 4 and 8, then row_bcast:15
+// into rows 1 and 3 and row_bcast:31 into rows 2 and 3.  Lane 63 holds the
+// total; it is read back as a wave-uniform value.
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xb1, 0xf, 0xf, false));  // quad_perm 1,0,3,2
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4e, 0xf, 0xf, false));  // quad_perm 2,3,0,1
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false)); // row_ror 4
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false)); // row_ror 8
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xa, 0xf, false)); // row_bcast:15
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xc, 0xf, false)); // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // Draws and trigonometry for one step (pair pick src/mc.cu:129-141; angles
@@ -168,65 +175,65 @@ __device__ __forceinline__ float field_lists(const SweepArgs& a, const float2* s
 
 // Local-field contribution of the thread for the current step, PAIRS layout
 // (fully connected, channel 0: f = a_p0 a_p1 conj(a_p2) conj(a_p3)).
-// For site x and others a<b<c: partner is a when o(b) > x, else c.
-// Rows are consumed PB at a time with all loads issued first (PB = the
-// rows per task of the work list, 8 or 16: one memory round trip per task).
+// For site x and others a<b<c the conjugation partner of x is a when
+// o(b) > x, else c.  With z = conj(s_c) W the pair's term is conj(s_b z') with
+// z' = conj(z) for partner a and z otherwise, so the segment accumulates
+// Hc = sum s_b z' and contributes -Re(d conj(Hc)).  Rows go PB at a time
+// with all loads issued first; row counts are multiples of 4, so every guard
+// is uniform across the wave.
 template <int PB>
-__device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s32, const uint32_t* pbc, int i0,
-                                             int j0, float2 di, float2 dj, int w, int lane) {
+__device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s32, int i0, int j0, float2 di,
+                                             float2 dj, int w, int lane) {
     float local = 0.f;
     const int t_end = a.wave_tasks[w + 1];
     for (int t = a.wave_tasks[w]; t < t_end; t++) {
         const int4 tk = a.tasks[t];
-        const int site = tk.x ? j0 : i0;
-        const float2 d = tk.x ? dj : di;
-        const int P0 = 4 * (tk.y + lane);
-        float2 U[4];
-        bool pa[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            U[j] = make_float2(0.f, 0.f);
-            pa[j] = true;
-            if (P0 + j < a.npairs) {
-                const uint32_t bc = pbc[P0 + j];
-                const int b = bc & 0xffff, c = bc >> 16;
-                const float2 tb = s32[b + (b >= site)], tc = s32[c + (c >= site)];
-                pa[j] = b >= site;
-                // partner a: U = conj(tb tc); partner c: U = conj(tb) tc
-                U[j] = pa[j] ? make_float2(tb.x * tc.x - tb.y * tc.y, -(tb.x * tc.y + tb.y * tc.x))
-                             : make_float2(tb.x * tc.x + tb.y * tc.y, tb.x * tc.y - tb.y * tc.x);
-            }
-        }
+        const int slot = tk.x & 0xffff, nr = tk.x >> 16;
+        const int site = slot ? j0 : i0;
+        const float2 d = slot ? dj : di;
+        const uint4 inf = *(const uint4*)(a.pinfo + (size_t)site * a.npad + tk.y + 4 * lane);
+        const float4* gp = (const float4*)(a.ptab + (size_t)site * a.slice_len + tk.w) + lane;
+        const float4* tp = (const float4*)(s32 + tk.z);
         float2 acc[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[j] = make_float2(0.f, 0.f);
-        const float* base = a.ptab + (size_t)site * a.slice_len + P0;
-        for (int r = tk.z; r < tk.w; r += PB) {
+        for (int r = 0; r < nr; r += PB) {
             float4 g[PB];
 #pragma unroll
-            for (int u = 0; u < PB; u++) {
-                const int ar = r + u;
-                g[u] = (ar < tk.w && P0 < a.row_len[ar]) ? *(const float4*)(base + a.row_off[ar])
-                                                         : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
+            for (int u = 0; u < PB; u += 4)
+                if (r + u < nr) {
 #pragma unroll
-            for (int u = 0; u < PB; u++) {
-                const int ar = min(r + u, tk.w - 1);
-                const float2 ta = s32[ar + (ar >= site)];
-                acc[0].x += g[u].x * ta.x; acc[0].y += g[u].x * ta.y;
-                acc[1].x += g[u].y * ta.x; acc[1].y += g[u].y * ta.y;
-                acc[2].x += g[u].z * ta.x; acc[2].y += g[u].z * ta.y;
-                acc[3].x += g[u].w * ta.x; acc[3].y += g[u].w * ta.y;
-            }
+                    for (int v = 0; v < 4; v++) g[u + v] = gp[(r + u + v) * 64];
+                }
+#pragma unroll
+            for (int u = 0; u < PB; u += 4)
+                if (r + u < nr) {
+                    const float4 t01 = tp[(r + u) / 2], t23 = tp[(r + u) / 2 + 1];
+                    const float ta[8] = {t01.x, t01.y, t01.z, t01.w, t23.x, t23.y, t23.z, t23.w};
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const float4 q = g[u + v];
+                        const float tx = ta[2 * v], ty = ta[2 * v + 1];
+                        acc[0].x += q.x * tx; acc[0].y += q.x * ty;
+                        acc[1].x += q.y * tx; acc[1].y += q.y * ty;
+                        acc[2].x += q.z * tx; acc[2].y += q.z * ty;
+                        acc[3].x += q.w * tx; acc[3].y += q.w * ty;
+                    }
+                }
         }
-        float2 H = make_float2(0.f, 0.f);
+        const uint32_t e[4] = {inf.x, inf.y, inf.z, inf.w};
+        float2 Hc = make_float2(0.f, 0.f);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const float2 W = pa[j] ? acc[j] : make_float2(acc[j].x, -acc[j].y);
-            H.x += U[j].x * W.x - U[j].y * W.y;
-            H.y += U[j].x * W.y + U[j].y * W.x;
+            const char* sb = (const char*)s32;
+            const float2 tb = *(const float2*)(sb + (e[j] & 0xffffu));
+            const float2 tc = *(const float2*)(sb + ((e[j] >> 16) & 0x7fffu));
+            const float zx = tc.x * acc[j].x + tc.y * acc[j].y;
+            const float zy = __uint_as_float(__float_as_uint(tc.x * acc[j].y - tc.y * acc[j].x) ^ (e[j] & 0x80000000u));
+            Hc.x += tb.x * zx - tb.y * zy;
+            Hc.y += tb.x * zy + tb.y * zx;
         }
-        local -= d.x * H.x - d.y * H.y;
+        local -= d.x * Hc.x + d.y * Hc.y;
     }
     return local;
 }
@@ -234,13 +241,12 @@ __device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s
 template <bool CUBE, bool PAIRS, bool BF16, int PB = 8>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     __shared__ double2 s64[kMaxN];
-    __shared__ float2 s32[kMaxN];
+    __shared__ __align__(16) float2 s32[kMaxN];
     __shared__ StepPre pre[kMaxN / 2];
     __shared__ float part[2][16];
     __shared__ double2 newv[2];
     __shared__ float2 dlt[2];
     __shared__ int pair[2];
-    __shared__ uint32_t pbc[PAIRS ? (kMaxPairsN - 1) * (kMaxPairsN - 2) / 2 : 1];
 
     const int rep = blockIdx.x;
     const int tid = threadIdx.x, bdim = blockDim.x;
@@ -255,8 +261,8 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         s64[i] = v;
         s32[i] = make_float2((float)v.x, (float)v.y);
     }
-    if (PAIRS)
-        for (int i = tid; i < a.npairs; i += bdim) pbc[i] = a.pair_bc[i];
+    if (PAIRS)  // pad rows of the tiles read these amplitudes with zero couplings
+        for (int i = N + tid; i < ((N + 3) & ~3); i += bdim) s32[i] = make_float2(0.f, 0.f);
     const double beta = a.betas[rep];
     const uint64_t seed = a.seeds[rep];
     const uint64_t sweep0 = a.ctr[rep];
@@ -344,7 +350,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 local = -(di.x * Fi.x - di.y * Fi.y) - (dj.x * Fj.x - dj.y * Fj.y);
             }
             if (PAIRS)
-                local += field_pairs<PB>(a, s32, pbc, i0, j0, di, dj, w, lane);
+                local += field_pairs<PB>(a, s32, i0, j0, di, dj, w, lane);
             else
                 local += field_lists<BF16, PB>(a, s32, i0, j0, di, dj, tid, bdim);
             local = wave_sum(local);
@@ -385,13 +391,9 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
     a.log_cap = st->log_cap;
     a.log_base = st->log_n;
     a.ptab = st->d_ptab;
+    a.pinfo = st->d_pinfo;
     a.slice_len = st->pair_slice_len;
-    for (int r = 0; r < (int)st->h_row_off.size(); r++) {
-        a.row_off[r] = st->h_row_off[r];
-        a.row_len[r] = st->h_row_len[r];
-    }
-    a.pair_bc = st->d_pair_bc;
-    a.npairs = st->pair_npairs;
+    a.npad = st->pair_npad;
     a.tasks = st->d_tasks;
     a.wave_tasks = st->d_wave_tasks;
     a.pair_groups = st->pair_groups;
