@@ -70,7 +70,7 @@ void free_all(pspin_state* st) {
     dfree(st->d_seeds); dfree(st->d_e2); dfree(st->d_e4); dfree(st->d_ctr);
     if (st->graph) (void)hipGraphExecDestroy(st->graph);
     st->graph = nullptr;
-    dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48);
+    dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48); dfree(st->d_done);
     dfree(st->d_series); dfree(st->d_series_n); dfree(st->d_log);
     dfree(st->d_ptab); dfree(st->d_pinfo);
     dfree(st->d_tasks); dfree(st->d_wave_tasks);
@@ -141,14 +141,18 @@ int build_site_lists(pspin_state* st, const std::vector<Term>& terms) {
 // tile boundary becomes several segments.  The largest run cost is minimised
 // by bisection over the cost bound with a greedy fill, where a segment costs
 // kSegCost plus kRowCost per row (instruction counts of the kernel's set-up
-// and row loop).  `rows` is the kernel's rows in flight per batch (8 or 16).
+// and row loop).  Wave 0 also computes the next step's proposal in FP64 and
+// starts with kPropCost.  Segments are then cut into batches of `rows` rows
+// (4 or 8), the kernel's unit of loads in flight.  Each 4-row quad of a batch
+// carries the number of lanes that hold a pair of its first row; the other
+// lanes' entries are zeros and are neither loaded nor multiplied.
 int build_pair_tasks(pspin_state* st, int nw_req, int rows) {
-    constexpr long long kSegCost = 64, kRowCost = 9;
+    constexpr long long kSegCost = 64, kRowCost = 9, kPropCost = 150;
     const int nblk = (int)st->h_tile_rows.size();
     struct Seg { int slot, k, y0, nr; };
     auto fill = [&](long long C, std::vector<std::vector<Seg>>* out) -> int {
         int waves = 1;
-        long long c = 0;
+        long long c = std::min(kPropCost, C - kSegCost - 4 * kRowCost);
         if (out) out->assign(1, {});
         for (int slot = 0; slot < 2; slot++)
             for (int k = 0; k < nblk; k++) {
@@ -184,11 +188,28 @@ int build_pair_tasks(pspin_state* st, int nw_req, int rows) {
     std::vector<std::vector<Seg>> per;
     fill(lo, &per);
     const int nw = (int)per.size();
+    const int pb = rows == 4 ? 4 : 8;
+    // Lanes holding a pair of row y (ext > y): a prefix, since b falls with the lane.
+    const std::vector<uint32_t>& pbc = st->h_pair_bc;
+    const int npairs = (int)pbc.size();
+    auto lanes_at = [&](int k, int y) {
+        int L = 0;
+        while (L < 64 && 256 * k + 4 * L < npairs && (int)(pbc[(size_t)256 * k + 4 * L] & 0xffff) + 1 > y) L++;
+        return L;
+    };
     std::vector<int4> flat;
     std::vector<int> wt(nw + 1, 0);
     for (int w = 0; w < nw; w++) {
+        int nb = 0;
         for (const Seg& g : per[w])
-            flat.push_back(make_int4(g.slot | (g.nr << 16), 256 * g.k, g.y0, st->h_tile_off[g.k] + 256 * g.y0));
+            for (int r = 0; r < g.nr; r += pb) {
+                const int n = std::min(pb, g.nr - r);
+                int x = g.slot | (r == 0 ? 2 : 0) | (r + n == g.nr ? 4 : 0) | (n << 8);
+                for (int q = 0; q < n / 4; q++) x |= lanes_at(g.k, g.y0 + r + 4 * q) << (12 + 7 * q);
+                flat.push_back(make_int4(x, 256 * g.k, g.y0 + r, st->h_tile_off[g.k] + 256 * (g.y0 + r)));
+                nb++;
+            }
+        PS_REQUIRE(nb <= 64, PSPIN_ERR_ARG, "pair-row work list: more than 64 batches for one wave (use more waves)");
         wt[w + 1] = (int)flat.size();
     }
     dfree(st->d_tasks);
@@ -198,7 +219,7 @@ int build_pair_tasks(pspin_state* st, int nw_req, int rows) {
     PS_TRY_HIP(cpy(st, st->d_tasks, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
     PS_TRY_HIP(cpy(st, st->d_wave_tasks, wt.data(), wt.size() * sizeof(int), hipMemcpyHostToDevice));
     st->pair_nwaves = nw;
-    st->pair_rows = rows == 16 ? 16 : 8;
+    st->pair_rows = pb;
     return PSPIN_OK;
 }
 
@@ -252,11 +273,12 @@ int build_pair_layout(pspin_state* st, const std::vector<double>& g4, const std:
     st->pair_npad = npad;
     st->h_tile_rows = trows;
     st->h_tile_off = toff;
+    st->h_pair_bc = pbc;
     PS_TRY_HIP(dmalloc(&st->d_ptab, tab.size()));
     PS_TRY_HIP(dmalloc(&st->d_pinfo, info.size()));
     PS_TRY_HIP(cpy(st, st->d_ptab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
     PS_TRY_HIP(cpy(st, st->d_pinfo, info.data(), info.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    return build_pair_tasks(st, 16, 16);
+    return build_pair_tasks(st, 16, 8);
 }
 
 // splitmix64 frequencies (src/mc.cu:449-460).
@@ -343,14 +365,14 @@ int harvest_events(pspin_state* st) {
     return PSPIN_OK;
 }
 
-int sweep_launch(pspin_state* st, int n) {
+int sweep_launch(pspin_state* st, int n, bool exchange = false) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (st->prof) {
         e0 = take_event(st);
         e1 = take_event(st);
         PS_TRY_HIP(hipEventRecord(e0, st->stream));
     }
-    PS_TRY_HIP(ps::launch_sweep(st, n));
+    PS_TRY_HIP(ps::launch_sweep(st, n, exchange));
     st->sweep_counter += (uint64_t)n;
     if (st->d_log) st->log_n = std::min(st->log_cap, st->log_n + n);
     if (st->prof) {
@@ -404,8 +426,7 @@ int ensure_graph(pspin_state* st, int pt_freq, int rounds) {
     PS_TRY_HIP(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int k = 0; k < rounds && e == hipSuccess; k++) {
-        e = ps::launch_sweep(st, pt_freq);
-        if (e == hipSuccess) e = ps::launch_exchange(st);
+        e = ps::launch_sweep(st, pt_freq, true);
     }
     hipError_t e2 = hipStreamEndCapture(st->stream, &g);
     PS_TRY_HIP(e);
@@ -458,13 +479,18 @@ int pt_advance(pspin_state* st, int n_sweeps, int pt_freq) {
         // Sweeps until the next exchange point ((s+1) % pt_freq == 0).
         long long to_next = pt_freq - (st->pt_sweeps % pt_freq);
         int chunk = (int)std::min<long long>(to_next, left);
-        int rc = sweep_launch(st, chunk);
+        // The exchange runs in the sweep launch unless the launch is timed on
+        // its own or the energies are resynchronised in between.
+        const bool at_exchange = (st->pt_sweeps + chunk) % pt_freq == 0;
+        const bool resync_due = st->resync_every > 0 && st->since_resync + chunk >= st->resync_every;
+        const bool fuse = at_exchange && !resync_due && !st->prof && ps::exchange_fusable(st);
+        int rc = sweep_launch(st, chunk, fuse);
         if (rc) return rc;
         st->pt_sweeps += chunk;
         done += chunk;
         st->since_resync += chunk;
         if ((rc = maybe_resync(st))) return rc;
-        if (st->pt_sweeps % pt_freq == 0) {
+        if (at_exchange && !fuse) {
             rc = exchange_launch(st);
             if (rc) return rc;
         }
@@ -880,7 +906,7 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
     PS_REQUIRE((size_t)NT * nrep_per_temp * 12 + (size_t)NT * 16 <= 60 * 1024, PSPIN_ERR_ARG,
                "too many replicas for the exchange kernel (NT * nrep_per_temp <= 5000)");
     drop_graph(st);
-    dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48);
+    dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48); dfree(st->d_done);
     st->pt_NT = NT;
     st->pt_nrep = nrep_per_temp;
     st->pt_sweeps = 0;
@@ -888,6 +914,8 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
     PS_TRY_HIP(dmalloc(&st->d_perm, (size_t)NT * nrep_per_temp));
     PS_TRY_HIP(dmalloc(&st->d_ex, (size_t)4 * (NT - 1)));
     PS_TRY_HIP(dmalloc(&st->d_rng48, 1));
+    PS_TRY_HIP(dmalloc(&st->d_done, 1));
+    PS_TRY_HIP(fill0(st, st->d_done, sizeof(unsigned int)));
     std::vector<int> perm((size_t)NT * nrep_per_temp);
     std::vector<double> betas((size_t)NT * nrep_per_temp);
     for (int i = 0; i < NT * nrep_per_temp; i++) {
@@ -1142,7 +1170,7 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
     std::vector<Cand> cands;
     const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
     if (st->d_ptab && !bf16 && st->variant == PSPIN_DENSE)
-        for (int rows : {8, 16})
+        for (int rows : {4, 8})
             for (int nw : {4, 8, 12, 16}) cands.push_back({PSPIN_KERNEL_PAIRS, nw, rows});
     for (int lb : {8, 16}) {
         if (lb == 16 && st->variant != PSPIN_DENSE) continue;
@@ -1159,7 +1187,12 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
     int rc = PSPIN_OK;
     for (const Cand& c : cands) {
         if (c.kernel == PSPIN_KERNEL_PAIRS) {
-            if ((rc = build_pair_tasks(st, c.param, c.rows))) break;
+            rc = build_pair_tasks(st, c.param, c.rows);
+            if (rc == PSPIN_ERR_ARG) {  // work list does not fit this shape
+                rc = PSPIN_OK;
+                continue;
+            }
+            if (rc) break;
         } else {
             st->block_size = c.param;
             st->list_batch = c.rows;
@@ -1222,8 +1255,8 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
 
 int pspin_set_pair_tiles(pspin_state* st, int nwaves, int rows) {
     PS_REQUIRE(st && st->d_ptab, PSPIN_ERR_STATE, "no pair-row layout (dense fully connected, 8 <= N <= 128)");
-    PS_REQUIRE(nwaves >= 1 && nwaves <= 16 && (rows == 8 || rows == 16), PSPIN_ERR_ARG,
-               "nwaves in [1, 16], rows 8 or 16");
+    PS_REQUIRE(nwaves >= 1 && nwaves <= 16 && (rows == 4 || rows == 8), PSPIN_ERR_ARG,
+               "nwaves in [1, 16], rows 4 or 8");
     drop_graph(st);
     return build_pair_tasks(st, nwaves, rows);
 }

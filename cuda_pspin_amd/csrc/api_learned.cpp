@@ -207,8 +207,7 @@ int ensure_graph(pspin_learned* lp, int pt_freq, int every) {
     LP_TRY_HIP(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int k = 0; every > 0 && k < every / pt_freq && e == hipSuccess; k++) {
-        e = ps::launch_sweep(st, pt_freq);
-        if (e == hipSuccess) e = ps::launch_exchange(st);
+        e = ps::launch_sweep(st, pt_freq, true);
     }
     if (e == hipSuccess) e = move_launch(lp, a);
     hipError_t e2 = hipStreamEndCapture(st->stream, &g);

@@ -8,35 +8,22 @@
 // u = drand48() and accept when u < exp(delta).  One wavefront serves 64
 // ladders per pass: a ballot over "needs a draw" gives each lane its rank in
 // the reference's draw order, and lcg48_skip jumps the 48-bit state to it.
-#include "pspin_internal.h"
+//
+// Up to 64 ladders (exchange_wave, run by k_pt_exchange or by the last
+// workgroup of a sweep launch): slots are staged in LDS once, lane r carries
+// ladder r's moving energy and replica down the levels in registers (level t
+// only swaps slots t and t+1, so slot t is final after level t and the values
+// of slot t+1 before it are the untouched ones), every lane computes the draw its rank would
+// take, X_{lane+1} = A_{lane+1} X + C_{lane+1}, lanes fetch their draw by rank
+// with one permute, and the state advances to the draw of lane (count - 1)
+// by a lane read: no LDS round trip on the chain from one level to the next.
+// More ladders (k_pt_exchange_wide) walk column blocks of 64 in LDS.
+#include "exchange_dev.h"
 
 namespace ps {
 
-struct ExArgs {
-    int NT, nrep;
-    const double* beta_sched;
-    const double* E;
-    int* perm;
-    double* betas;
-    long long* ex;  // [4][NT-1]: acc_win, prop_win, acc_tot, prop_tot
-    uint64_t* rng48;
-    double* series;
-    long long* series_n;
-    int series_cap;
-};
 
-// Accept test u < exp(delta) for delta < 0, decided exactly as in FP64: an
-// FP32 exponential (relative error well under 1e-5 over the whole range,
-// flushing to 0 below about -87) settles every u outside a 2e-5 relative
-// band around it; inside the band the FP64 exp decides.
-__device__ __forceinline__ bool accept_below(double u, double delta) {
-    const double e32 = (double)__expf((float)delta);
-    if (u < e32 * (1.0 - 2e-5)) return true;
-    if (u > e32 * (1.0 + 2e-5)) return false;
-    return u < exp(delta);
-}
-
-__global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
+__global__ __launch_bounds__(64) void k_pt_exchange_wide(ExArgs a) {
     // LDS: the permutation and the energies by slot (sEp[s] = E[perm[s]],
     // swapped together), betas, per-pair accept counts.  Lane r owns ladder
     // column r at every level, so levels need no barrier.
@@ -118,7 +105,14 @@ __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
     }
 }
 
-hipError_t launch_exchange(pspin_state* st) {
+__global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
+    extern __shared__ double lds_ex[];
+    exchange_stage(a, lds_ex, threadIdx.x, 64);
+    __syncthreads();
+    exchange_wave(a, lds_ex, threadIdx.x);
+}
+
+ExArgs exchange_args(const pspin_state* st) {
     ExArgs a;
     a.NT = st->pt_NT;
     a.nrep = st->pt_nrep;
@@ -131,8 +125,17 @@ hipError_t launch_exchange(pspin_state* st) {
     a.series = st->series_cap > 0 ? st->d_series : nullptr;
     a.series_n = st->d_series_n;
     a.series_cap = st->series_cap;
-    size_t lds = (size_t)a.NT * a.nrep * (sizeof(double) + sizeof(int)) + (size_t)a.NT * (sizeof(double) + 8);
-    hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), lds, st->stream, a);
+    return a;
+}
+
+hipError_t launch_exchange(pspin_state* st) {
+    const ExArgs a = exchange_args(st);
+    if (a.nrep <= 64) {
+        hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), exchange_lds_bytes(a.NT, a.nrep), st->stream, a);
+    } else {
+        size_t lds = (size_t)a.NT * a.nrep * (sizeof(double) + sizeof(int)) + (size_t)a.NT * (sizeof(double) + 8);
+        hipLaunchKernelGGL(k_pt_exchange_wide, dim3(1), dim3(64), lds, st->stream, a);
+    }
     return hipGetLastError();
 }
 

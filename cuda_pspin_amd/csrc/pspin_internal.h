@@ -200,6 +200,7 @@ struct pspin_state {
     int pair_npairs = 0, pair_npad = 0;
     std::vector<int> h_tile_rows;  // rows of pair block k's tile (multiple of 4)
     std::vector<int> h_tile_off;   // offset of pair block k's tile in a slice (floats)
+    std::vector<uint32_t> h_pair_bc;  // pairs b | c << 16 (others relabelled), b descending
     int4* d_tasks = nullptr;
     int* d_wave_tasks = nullptr;
     int pair_nwaves = 0;
@@ -235,6 +236,7 @@ struct pspin_state {
     double* d_series = nullptr;
     int series_cap = 0;
     long long* d_series_n = nullptr;
+    unsigned int* d_done = nullptr;  // workgroups finished in the current sweep launch (last one exchanges)
 
     // step log
     ps::StepRec* d_log = nullptr;
@@ -271,7 +273,31 @@ hipError_t launch_energy_dense(const pspin_state* st, double* e2, double* e4);
 hipError_t launch_energy_sparse(const pspin_state* st, double* e2, double* e4);
 hipError_t launch_add(const double* a, const double* b, double* out, int n, hipStream_t s);
 
-hipError_t launch_sweep(pspin_state* st, int n_sweeps);
+// Replica-exchange arguments (exchange.hip), also carried by a sweep launch
+// that runs the exchange in its last workgroup.
+struct ExArgs {
+    int NT, nrep;
+    const double* beta_sched;
+    const double* E;
+    int* perm;
+    double* betas;
+    long long* ex;  // [4][NT-1]: acc_win, prop_win, acc_tot, prop_tot
+    uint64_t* rng48;
+    double* series;
+    long long* series_n;
+    int series_cap;
+};
+ExArgs exchange_args(const pspin_state* st);
+// LDS bytes of the one-wave exchange for NT x nrep slots.
+inline size_t exchange_lds_bytes(int NT, int nrep) {
+    return (size_t)NT * nrep * (sizeof(double) + sizeof(int)) + (size_t)NT * 2 * sizeof(double);
+}
+// Whether a sweep launch can run the exchange in its last workgroup.
+bool exchange_fusable(const pspin_state* st);
+
+// Sweeps; with exchange = true the replica exchange follows (fused into the
+// same launch when exchange_fusable, else a separate launch).
+hipError_t launch_sweep(pspin_state* st, int n_sweeps, bool exchange = false);
 
 hipError_t launch_exchange(pspin_state* st);
 

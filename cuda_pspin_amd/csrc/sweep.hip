@@ -42,12 +42,13 @@
 // steps drawn in parallel at sweep start.  Proposal arithmetic and the master
 // amplitudes are FP64 (the constraint is kept to FP64 rounding, as in
 // the reference); Delta E is FP32; the running energy is FP64.
-#include "pspin_internal.h"
+#include "exchange_dev.h"
 
 namespace ps {
 
 struct SweepArgs {
     int N, nrep, h2_active, n_sweeps;
+    int g2_lds;             // the FP32 two-body matrix is staged in dynamic LDS
     int pair_groups;        // 0: pairs from the replica stream; G > 0: from group (rep % G)'s stream
     uint64_t pair_seed;     // run seed the group keys derive from
     double* spins;
@@ -68,8 +69,12 @@ struct SweepArgs {
     const uint32_t* pinfo;    // [N][npad] per-site pair amplitude offsets and partner flags
     long long slice_len;
     int npad;
-    const int4* tasks;        // {slot | rows << 16, first pair, first row y0, tile offset of row y0}
-    const int* wave_tasks;    // [nwaves+1] segment ranges per wave
+    // replica exchange by the last workgroup to finish (exchange != 0)
+    int exchange;
+    unsigned int* done;
+    ExArgs ex;
+    const int4* tasks;        // row batches {slot | first << 1 | last << 2 | rows << 8, first pair, first row, tile offset}
+    const int* wave_tasks;    // [nwaves+1] batch ranges per wave (at most 64 each)
 };
 
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
@@ -177,76 +182,161 @@ __device__ __forceinline__ float field_lists(const SweepArgs& a, const float2* s
 // (fully connected, channel 0: f = a_p0 a_p1 conj(a_p2) conj(a_p3)).
 // For site x and others a<b<c the conjugation partner of x is a when
 // o(b) > x, else c.  With z = conj(s_c) W the pair's term is conj(s_b z') with
-// z' = conj(z) for partner a and z otherwise, so the segment accumulates
-// Hc = sum s_b z' and contributes -Re(d conj(Hc)).  Rows go PB at a time
-// with all loads issued first; row counts are multiples of 4, so every guard
-// is uniform across the wave.
+// z' = conj(z) for partner a and z otherwise, so a segment accumulates
+// Hc = sum s_b z' and contributes -Re(d conj(Hc)).
+//
+// A wave's work is a static list of batches of at most PB rows (a multiple
+// of 4), descriptor
+//   {slot | first << 1 | last << 2 | rows << 8 | lanes(quad q) << (12 + 7 q),
+//    first pair, first row, tile offset},
+// held one per lane.  lanes(q) is the number of lanes (a prefix) that hold a
+// pair of the quad's rows; only they load and multiply it.  Loads run one batch ahead of the arithmetic in one
+// register buffer: as soon as a quad of rows has been multiplied, its
+// registers take the next batch's loads for the same quad.  The last batch of
+// a step prefetches the next step's first batch (its sites are drawn at sweep
+// start), so those loads are in flight across the barrier.
 template <int PB>
-__device__ __forceinline__ float field_pairs(const SweepArgs& a, const float2* s32, int i0, int j0, float2 di,
-                                             float2 dj, int w, int lane) {
-    float local = 0.f;
-    const int t_end = a.wave_tasks[w + 1];
-    for (int t = a.wave_tasks[w]; t < t_end; t++) {
-        const int4 tk = a.tasks[t];
-        const int slot = tk.x & 0xffff, nr = tk.x >> 16;
-        const int site = slot ? j0 : i0;
-        const float2 d = slot ? dj : di;
-        const uint4 inf = *(const uint4*)(a.pinfo + (size_t)site * a.npad + tk.y + 4 * lane);
-        const float4* gp = (const float4*)(a.ptab + (size_t)site * a.slice_len + tk.w) + lane;
-        const float4* tp = (const float4*)(s32 + tk.z);
-        float2 acc[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[j] = make_float2(0.f, 0.f);
-        for (int r = 0; r < nr; r += PB) {
-            float4 g[PB];
-#pragma unroll
-            for (int u = 0; u < PB; u += 4)
-                if (r + u < nr) {
-#pragma unroll
-                    for (int v = 0; v < 4; v++) g[u + v] = gp[(r + u + v) * 64];
-                }
-#pragma unroll
-            for (int u = 0; u < PB; u += 4)
-                if (r + u < nr) {
-                    const float4 t01 = tp[(r + u) / 2], t23 = tp[(r + u) / 2 + 1];
-                    const float ta[8] = {t01.x, t01.y, t01.z, t01.w, t23.x, t23.y, t23.z, t23.w};
-#pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const float4 q = g[u + v];
-                        const float tx = ta[2 * v], ty = ta[2 * v + 1];
-                        acc[0].x += q.x * tx; acc[0].y += q.x * ty;
-                        acc[1].x += q.y * tx; acc[1].y += q.y * ty;
-                        acc[2].x += q.z * tx; acc[2].y += q.z * ty;
-                        acc[3].x += q.w * tx; acc[3].y += q.w * ty;
-                    }
-                }
-        }
-        const uint32_t e[4] = {inf.x, inf.y, inf.z, inf.w};
-        float2 Hc = make_float2(0.f, 0.f);
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const char* sb = (const char*)s32;
-            const float2 tb = *(const float2*)(sb + (e[j] & 0xffffu));
-            const float2 tc = *(const float2*)(sb + ((e[j] >> 16) & 0x7fffu));
-            const float zx = tc.x * acc[j].x + tc.y * acc[j].y;
-            const float zy = __uint_as_float(__float_as_uint(tc.x * acc[j].y - tc.y * acc[j].x) ^ (e[j] & 0x80000000u));
-            Hc.x += tb.x * zx - tb.y * zy;
-            Hc.y += tb.x * zy + tb.y * zx;
-        }
-        local -= d.x * Hc.x + d.y * Hc.y;
-    }
-    return local;
+struct PairBuf {
+    float4 g[PB];
+    uint4 inf;  // pair-info words of the buffered batch (first batch of a segment)
+};
+
+__device__ __forceinline__ int4 desc_at(int4 d, int b) {
+    return make_int4(__builtin_amdgcn_readlane(d.x, b), __builtin_amdgcn_readlane(d.y, b),
+                     __builtin_amdgcn_readlane(d.z, b), __builtin_amdgcn_readlane(d.w, b));
 }
 
+__device__ __forceinline__ int quad_lanes(int x, int q) { return (x >> (12 + 7 * q)) & 127; }
+
+// Loads of rows [u0, u0 + 4) of batch bd for the given pair site (lanes
+// holding pairs of those rows only; the others neither load nor multiply).
+template <int PB>
+__device__ __forceinline__ void pair_load_quad(const SweepArgs& a, PairBuf<PB>& B, int4 bd, int site, int lane, int u0) {
+    if (lane < quad_lanes(bd.x, u0 / 4)) {
+        const float4* gp = (const float4*)(a.ptab + (size_t)site * a.slice_len + bd.w) + lane;
+#pragma unroll
+        for (int v = 0; v < 4; v++) B.g[u0 + v] = gp[(u0 + v) * 64];
+    }
+}
+
+template <int PB>
+__device__ __forceinline__ void pair_issue(const SweepArgs& a, PairBuf<PB>& B, int4 bd, int i0, int j0, int lane) {
+    const int nr = (bd.x >> 8) & 15;
+    const int site = (bd.x & 1) ? j0 : i0;
+    if (bd.x & 2) B.inf = *(const uint4*)(a.pinfo + (size_t)site * a.npad + bd.y + 4 * lane);
+#pragma unroll
+    for (int u = 0; u < PB; u += 4)
+        if (u < nr) pair_load_quad<PB>(a, B, bd, site, lane, u);
+}
+
+// Multiplies buffered batch bd into acc while loading batch bn (sites ni, nj;
+// bn.x == 0 loads nothing), and returns the segment's contribution when bd is
+// its last batch.
+template <int PB>
+__device__ __forceinline__ float pair_step(const SweepArgs& a, PairBuf<PB>& B, int4 bd, int4 bn, int ni, int nj,
+                                           const float2* s32, float2 di, float2 dj, float2 (&acc)[4], uint4& inf,
+                                           int lane) {
+    const int nr = (bd.x >> 8) & 15, nn = (bn.x >> 8) & 15;
+    const int nsite = (bn.x & 1) ? nj : ni;
+    if (bd.x & 2) {
+        inf = B.inf;
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[j] = make_float2(0.f, 0.f);
+    }
+    if (bn.x & 2) B.inf = *(const uint4*)(a.pinfo + (size_t)nsite * a.npad + bn.y + 4 * lane);
+    const float4* tp = (const float4*)(s32 + bd.z);
+#pragma unroll
+    for (int u = 0; u < PB; u += 4) {
+        if (u < nr && lane < quad_lanes(bd.x, u / 4)) {
+            const float4 t01 = tp[u / 2], t23 = tp[u / 2 + 1];
+            const float ta[8] = {t01.x, t01.y, t01.z, t01.w, t23.x, t23.y, t23.z, t23.w};
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const float4 q = B.g[u + v];
+                const float tx = ta[2 * v], ty = ta[2 * v + 1];
+                acc[0].x += q.x * tx; acc[0].y += q.x * ty;
+                acc[1].x += q.y * tx; acc[1].y += q.y * ty;
+                acc[2].x += q.z * tx; acc[2].y += q.z * ty;
+                acc[3].x += q.w * tx; acc[3].y += q.w * ty;
+            }
+        }
+        if (u < nn) pair_load_quad<PB>(a, B, bn, nsite, lane, u);
+    }
+    if (!(bd.x & 4)) return 0.f;
+    const uint32_t e[4] = {inf.x, inf.y, inf.z, inf.w};
+    float2 Hc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const char* sb = (const char*)s32;
+        const float2 tb = *(const float2*)(sb + (e[j] & 0xffffu));
+        const float2 tc = *(const float2*)(sb + ((e[j] >> 16) & 0x7fffu));
+        const float zx = tc.x * acc[j].x + tc.y * acc[j].y;
+        const float zy = __uint_as_float(__float_as_uint(tc.x * acc[j].y - tc.y * acc[j].x) ^ (e[j] & 0x80000000u));
+        Hc.x += tb.x * zx - tb.y * zy;
+        Hc.y += tb.x * zy + tb.y * zx;
+    }
+    const float2 d = (bd.x & 1) ? dj : di;
+    return -(d.x * Hc.x + d.y * Hc.y);
+}
+
+// A proposal for one step under one hypothesis about the step before it.
+struct Prop {
+    double2 ni, nj;      // proposed FP64 amplitudes
+    float2 oi, oj;       // FP32 of the current amplitudes
+    float2 fi, fj;       // FP32 of the proposed amplitudes
+    float2 mi, mj;       // FP32 midpoints
+    float2 di, dj;       // FP32 differences new - old
+};
+
+// Constraint-preserving proposal (include/spins.h:17-52,
+// include/spins_sparse.h:26-56), FP64, from the current amplitudes ai, aj.
+template <bool CUBE>
+__device__ __forceinline__ Prop propose(const StepPre& P, double2 ai, double2 aj) {
+    const double r1 = ai.x * ai.x + ai.y * ai.y, r2 = aj.x * aj.x + aj.y * aj.y;
+    double2 ni, nj;
+    if (CUBE) {
+        const double sq = sqrt(r1 * r1 + r2 * r2);
+        const double m1 = sqrt(sq * P.c[0]), m2 = sqrt(sq * P.c[1]);
+        ni = make_double2(m1 * P.c[2], m1 * P.c[3]);
+        nj = make_double2(m2 * P.c[4], m2 * P.c[5]);
+    } else {
+        const double f1 = sqrt((r1 + r2) / r1) * P.c[0], f2 = sqrt((r1 + r2) / r2) * P.c[1];
+        ni = make_double2((ai.x * P.c[2] + ai.y * P.c[3]) * f1, (-ai.x * P.c[3] + ai.y * P.c[2]) * f1);
+        nj = make_double2((aj.x * P.c[4] + aj.y * P.c[5]) * f2, (-aj.x * P.c[5] + aj.y * P.c[4]) * f2);
+    }
+    Prop q;
+    q.ni = ni;
+    q.nj = nj;
+    q.oi = make_float2((float)ai.x, (float)ai.y);
+    q.oj = make_float2((float)aj.x, (float)aj.y);
+    q.fi = make_float2((float)ni.x, (float)ni.y);
+    q.fj = make_float2((float)nj.x, (float)nj.y);
+    q.mi = make_float2((float)(0.5 * (ai.x + ni.x)), (float)(0.5 * (ai.y + ni.y)));
+    q.mj = make_float2((float)(0.5 * (aj.x + nj.x)), (float)(0.5 * (aj.y + nj.y)));
+    q.di = make_float2((float)(ni.x - ai.x), (float)(ni.y - ai.y));
+    q.dj = make_float2((float)(nj.x - aj.x), (float)(nj.y - aj.y));
+    return q;
+}
+
+// Step pipeline, one workgroup barrier per step.  During step t's field
+// phase, lanes 0 and 1 of wave 0 compute step t+1's proposal under the two
+// outcomes of step t (rejected, accepted): the pairs of consecutive steps may
+// share a site, and then the proposal depends on the outcome.  After the
+// barrier every wave sums the partial Delta E of step t, decides acceptance
+// (the same FP32 sum and FP64 threshold in every wave), and updates its own
+// FP32 copy of the amplitudes: the final values of step t's sites, then the
+// midpoints of step t+1's.  Private copies are written with wave-uniform
+// addresses and values by all lanes, so each wave's LDS program order is the
+// only ordering needed.  Wave 0 owns the FP64 amplitudes.
 template <bool CUBE, bool PAIRS, bool BF16, int PB = 8>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     __shared__ double2 s64[kMaxN];
-    __shared__ __align__(16) float2 s32[kMaxN];
+    __shared__ __align__(16) float2 view[16][kMaxN];
     __shared__ StepPre pre[kMaxN / 2];
     __shared__ float part[2][16];
-    __shared__ double2 newv[2];
-    __shared__ float2 dlt[2];
-    __shared__ int pair[2];
+    __shared__ Prop prop[3][2];  // [step mod 3][outcome of the step before]
+    __shared__ int last_wg;
+    extern __shared__ float g2s[];  // [N][N] two-body couplings when g2_lds
 
     const int rep = blockIdx.x;
     const int tid = threadIdx.x, bdim = blockDim.x;
@@ -255,14 +345,12 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     const int nw = bdim >> 6, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int N = a.N, nsteps = N / 2;
     double* gs = a.spins + (size_t)2 * N * rep;
+    float2* s32 = view[w];
 
-    for (int i = tid; i < N; i += bdim) {
-        double2 v = make_double2(gs[2 * i], gs[2 * i + 1]);
-        s64[i] = v;
-        s32[i] = make_float2((float)v.x, (float)v.y);
-    }
-    if (PAIRS)  // pad rows of the tiles read these amplitudes with zero couplings
-        for (int i = N + tid; i < ((N + 3) & ~3); i += bdim) s32[i] = make_float2(0.f, 0.f);
+    for (int i = tid; i < N; i += bdim) s64[i] = make_double2(gs[2 * i], gs[2 * i + 1]);
+    // pad entries are read by zero-coupling rows of the pair tiles
+    for (int i = lane; i < ((N + 3) & ~3); i += 64)
+        s32[i] = i < N ? make_float2((float)gs[2 * i], (float)gs[2 * i + 1]) : make_float2(0.f, 0.f);
     const double beta = a.betas[rep];
     const uint64_t seed = a.seeds[rep];
     const uint64_t sweep0 = a.ctr[rep];
@@ -270,93 +358,123 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     double E = 0.0;
     long long nacc = 0, nprop = 0;
     if (tid == 0) E = a.E[rep];
+    if (a.h2_active && a.g2_lds)
+        for (int i = tid; i < N * N; i += bdim) g2s[i] = a.g2[i];
+
+    // PAIRS: this wave's batch descriptors, one per lane (an even count)
+    int4 desc = make_int4(0, 0, 0, 0);
+    int nb = 0;
+    PairBuf<PB> bA;
+    if (PAIRS) {
+        const int t0 = a.wave_tasks[w];
+        nb = a.wave_tasks[w + 1] - t0;
+        if (lane < nb) desc = a.tasks[t0 + lane];
+    }
 
     for (int sw = 0; sw < a.n_sweeps; sw++) {
         const uint64_t sweep = sweep0 + (uint64_t)sw;
         for (int k = tid; k < nsteps; k += bdim) pre[k] = make_step<CUBE>(seed, gkey, sweep, k, N, beta);
         __syncthreads();
+        if (PAIRS && nb > 0) pair_issue<PB>(a, bA, desc_at(desc, 0), pre[0].i0, pre[0].j0, lane);
         StepRec* logp = nullptr;
         if (a.log && a.log_base + sw < a.log_cap)
             logp = a.log + ((size_t)rep * a.log_cap + a.log_base + sw) * nsteps;
+        if (w == 0 && lane < 2) {
+            const StepPre& P = pre[0];
+            prop[0][lane] = propose<CUBE>(P, s64[P.i0], s64[P.j0]);
+        }
+        __syncthreads();
 
+        int hyp = 0;  // outcome of the step before the current one (selects its proposal)
         for (int step = 0; step <= nsteps; step++) {
-            if (tid == 0) {
-                if (step > 0) {
-                    // Resolve the previous proposal (src/mc.cu:284-304).
-                    const int sp = step - 1, pi = pair[0], pj = pair[1];
-                    float dE = 0.f;
-                    for (int k = 0; k < nw; k++) dE += part[sp & 1][k];
-                    const bool accept = (beta == 0.0) || ((double)dE < pre[sp].thr);
+            if (step > 0) {
+                // Resolve step-1 (src/mc.cu:284-304).
+                const int sp = step - 1;
+                const StepPre& Pp = pre[sp];
+                const Prop& Q = prop[sp % 3][hyp];
+                float dE = 0.f;
+                for (int k = 0; k < nw; k++) dE += part[sp & 1][k];
+                const bool accept = (beta == 0.0) || ((double)dE < Pp.thr);
+                s32[Pp.i0] = accept ? Q.fi : Q.oi;
+                s32[Pp.j0] = accept ? Q.fj : Q.oj;
+                if (w == 0 && accept) {
+                    s64[Pp.i0] = Q.ni;
+                    s64[Pp.j0] = Q.nj;
+                }
+                if (tid == 0) {
                     if (accept) {
-                        s64[pi] = newv[0];
-                        s64[pj] = newv[1];
                         E += (double)dE;
                         nacc++;
                     }
-                    s32[pi] = make_float2((float)s64[pi].x, (float)s64[pi].y);
-                    s32[pj] = make_float2((float)s64[pj].x, (float)s64[pj].y);
                     nprop++;
                     if (logp) {
                         StepRec r;
-                        r.i0 = pi; r.j0 = pj; r.acc = accept; r.dE = dE;
-                        r.ai[0] = newv[0].x; r.ai[1] = newv[0].y;
-                        r.aj[0] = newv[1].x; r.aj[1] = newv[1].y;
+                        r.i0 = Pp.i0; r.j0 = Pp.j0; r.acc = accept; r.dE = dE;
+                        r.ai[0] = Q.ni.x; r.ai[1] = Q.ni.y;
+                        r.aj[0] = Q.nj.x; r.aj[1] = Q.nj.y;
                         logp[sp] = r;
                     }
                 }
-                if (step < nsteps) {
-                    // Constraint-preserving proposal (include/spins.h:17-52,
-                    // include/spins_sparse.h:26-56), FP64.
-                    const StepPre& P = pre[step];
-                    const int i0 = P.i0, j0 = P.j0;
-                    const double2 ai = s64[i0], aj = s64[j0];
-                    const double r1 = ai.x * ai.x + ai.y * ai.y, r2 = aj.x * aj.x + aj.y * aj.y;
-                    double2 ni, nj;
-                    if (CUBE) {
-                        const double sq = sqrt(r1 * r1 + r2 * r2);
-                        const double m1 = sqrt(sq * P.c[0]), m2 = sqrt(sq * P.c[1]);
-                        ni = make_double2(m1 * P.c[2], m1 * P.c[3]);
-                        nj = make_double2(m2 * P.c[4], m2 * P.c[5]);
-                    } else {
-                        const double f1 = sqrt((r1 + r2) / r1) * P.c[0], f2 = sqrt((r1 + r2) / r2) * P.c[1];
-                        ni = make_double2((ai.x * P.c[2] + ai.y * P.c[3]) * f1, (-ai.x * P.c[3] + ai.y * P.c[2]) * f1);
-                        nj = make_double2((aj.x * P.c[4] + aj.y * P.c[5]) * f2, (-aj.x * P.c[5] + aj.y * P.c[4]) * f2);
-                    }
-                    newv[0] = ni;
-                    newv[1] = nj;
-                    dlt[0] = make_float2((float)(ni.x - ai.x), (float)(ni.y - ai.y));
-                    dlt[1] = make_float2((float)(nj.x - aj.x), (float)(nj.y - aj.y));
-                    s32[i0] = make_float2((float)(0.5 * (ai.x + ni.x)), (float)(0.5 * (ai.y + ni.y)));
-                    s32[j0] = make_float2((float)(0.5 * (aj.x + nj.x)), (float)(0.5 * (aj.y + nj.y)));
-                    pair[0] = i0;
-                    pair[1] = j0;
-                }
+                hyp = accept;
             }
-            __syncthreads();
             if (step == nsteps) break;
 
-            const int i0 = __builtin_amdgcn_readfirstlane(pair[0]), j0 = __builtin_amdgcn_readfirstlane(pair[1]);
-            const float2 di = dlt[0], dj = dlt[1];
+            const StepPre& P = pre[step];
+            const Prop& Q = prop[step % 3][hyp];
+            const int i0 = P.i0, j0 = P.j0;
+            const float2 di = Q.di, dj = Q.dj;
+            s32[i0] = Q.mi;
+            s32[j0] = Q.mj;
             float local = 0.f;
-            if (a.h2_active) {
+            if (a.h2_active && w == nw - 1) {
                 float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
-                const float* ri = a.g2 + (size_t)i0 * N;
-                const float* rj = a.g2 + (size_t)j0 * N;
-                for (int k = tid; k < N; k += bdim) {
-                    const float2 t = s32[k];
-                    if (k != i0) { const float g = ri[k]; Fi.x += g * t.x; Fi.y -= g * t.y; }
-                    if (k != j0) { const float g = rj[k]; Fj.x += g * t.x; Fj.y -= g * t.y; }
+                if (a.g2_lds) {  // no global load here: it would wait behind the prefetched rows
+                    for (int k = lane; k < N; k += 64) {
+                        const float2 t = s32[k];
+                        const float gi = k != i0 ? g2s[i0 * N + k] : 0.f, gj = k != j0 ? g2s[j0 * N + k] : 0.f;
+                        Fi.x += gi * t.x; Fi.y -= gi * t.y;
+                        Fj.x += gj * t.x; Fj.y -= gj * t.y;
+                    }
+                } else {
+                    const float* ri = a.g2 + (size_t)i0 * N;
+                    const float* rj = a.g2 + (size_t)j0 * N;
+                    for (int k = lane; k < N; k += 64) {
+                        const float2 t = s32[k];
+                        if (k != i0) { const float g = ri[k]; Fi.x += g * t.x; Fi.y -= g * t.y; }
+                        if (k != j0) { const float g = rj[k]; Fj.x += g * t.x; Fj.y -= g * t.y; }
+                    }
                 }
                 local = -(di.x * Fi.x - di.y * Fi.y) - (dj.x * Fj.x - dj.y * Fj.y);
             }
-            if (PAIRS)
-                local += field_pairs<PB>(a, s32, i0, j0, di, dj, w, lane);
-            else
+            if (PAIRS) {
+                float2 acc[4];
+                uint4 inf = make_uint4(0u, 0u, 0u, 0u);
+                const int ni = step + 1 < nsteps ? pre[step + 1].i0 : 0, nj = step + 1 < nsteps ? pre[step + 1].j0 : 0;
+                for (int b = 0; b < nb; b++) {
+                    const int4 bd = desc_at(desc, b);
+                    int4 bn = make_int4(0, 0, 0, 0);
+                    if (b + 1 < nb) bn = desc_at(desc, b + 1);
+                    else if (step + 1 < nsteps) bn = desc_at(desc, 0);
+                    const bool same = b + 1 < nb;
+                    local += pair_step<PB>(a, bA, bd, bn, same ? i0 : ni, same ? j0 : nj, s32, di, dj, acc, inf, lane);
+                }
+            } else
                 local += field_lists<BF16, PB>(a, s32, i0, j0, di, dj, tid, bdim);
             local = wave_sum(local);
             if (lane == 0) part[step & 1][w] = local;
+            if (w == 0 && lane < 2 && step + 1 < nsteps) {
+                // Next step's proposal if this step is rejected (lane 0) or accepted (lane 1).
+                const StepPre& Pn = pre[step + 1];
+                double2 ai = s64[Pn.i0], aj = s64[Pn.j0];
+                if (lane == 1) {
+                    if (Pn.i0 == i0) ai = Q.ni; else if (Pn.i0 == j0) ai = Q.nj;
+                    if (Pn.j0 == i0) aj = Q.ni; else if (Pn.j0 == j0) aj = Q.nj;
+                }
+                prop[(step + 1) % 3][lane] = propose<CUBE>(Pn, ai, aj);
+            }
             __syncthreads();
         }
+        __syncthreads();  // pre[] and prop[] are rewritten by the next sweep
     }
 
     for (int i = tid; i < N; i += bdim) {
@@ -369,13 +487,40 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         a.prop[rep] += nprop;
         a.ctr[rep] = sweep0 + (uint64_t)a.n_sweeps;
     }
+    if (a.exchange) {
+        // Last workgroup out runs the replica exchange on every energy.
+        if (tid == 0) {
+            __threadfence();
+            last_wg = atomicAdd(a.done, 1u) == gridDim.x - 1;
+        }
+        __syncthreads();
+        if (last_wg) {
+            __threadfence();
+            if (tid == 0) *a.done = 0u;
+            exchange_stage(a.ex, &view[0][0], tid, bdim);
+            __syncthreads();
+            if (w == 0) exchange_wave(a.ex, &view[0][0], lane);
+        }
+    }
 }
 
-hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
+constexpr size_t kG2LdsMax = 64 * 1024;  // two-body matrix staged in LDS up to N = 128
+
+bool exchange_fusable(const pspin_state* st) {
+    return st->pt_NT >= 2 && st->pt_nrep <= 64 && st->d_done &&
+           exchange_lds_bytes(st->pt_NT, st->pt_nrep) <= sizeof(float2) * 16 * kMaxN;
+}
+
+hipError_t launch_sweep(pspin_state* st, int n_sweeps, bool exchange) {
     SweepArgs a;
+    a.exchange = exchange && exchange_fusable(st);
+    a.done = st->d_done;
+    if (a.exchange) a.ex = exchange_args(st);
     a.N = st->N;
     a.nrep = st->nrep;
     a.h2_active = st->h2_active;
+    a.g2_lds = st->h2_active && (size_t)st->N * st->N * sizeof(float) <= kG2LdsMax;
+    const size_t shm = a.g2_lds ? (size_t)st->N * st->N * sizeof(float) : 0;
     a.n_sweeps = n_sweeps;
     a.spins = st->d_spins;
     a.g2 = st->d_g2_32;
@@ -408,21 +553,23 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps) {
     const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
     const bool lb16 = st->list_batch == 16;
     if (st->variant == PSPIN_SPARSE) {
-        if (bf16) hipLaunchKernelGGL((k_sweep<true, false, true>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
-        else hipLaunchKernelGGL((k_sweep<true, false, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        if (bf16) hipLaunchKernelGGL((k_sweep<true, false, true>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
+        else hipLaunchKernelGGL((k_sweep<true, false, false>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
     } else if (pairs) {
-        if (st->pair_rows == 16)
-            hipLaunchKernelGGL((k_sweep<false, true, false, 16>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        if (st->pair_rows == 4)
+            hipLaunchKernelGGL((k_sweep<false, true, false, 4>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
         else
-            hipLaunchKernelGGL((k_sweep<false, true, false, 8>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+            hipLaunchKernelGGL((k_sweep<false, true, false, 8>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
     } else if (lb16) {
-        if (bf16) hipLaunchKernelGGL((k_sweep<false, false, true, 16>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
-        else hipLaunchKernelGGL((k_sweep<false, false, false, 16>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        if (bf16) hipLaunchKernelGGL((k_sweep<false, false, true, 16>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
+        else hipLaunchKernelGGL((k_sweep<false, false, false, 16>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
     } else {
-        if (bf16) hipLaunchKernelGGL((k_sweep<false, false, true>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
-        else hipLaunchKernelGGL((k_sweep<false, false, false>), dim3(st->nrep), dim3(bs), 0, st->stream, a);
+        if (bf16) hipLaunchKernelGGL((k_sweep<false, false, true>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
+        else hipLaunchKernelGGL((k_sweep<false, false, false>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && exchange && !a.exchange) e = launch_exchange(st);
+    return e;
 }
 
 }  // namespace ps

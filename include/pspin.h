@@ -241,7 +241,9 @@ int pspin_set_kernel(pspin_state* st, int kernel, int block_size);
  * gives each replica's group key (0 with per-replica pairs). */
 int pspin_set_pair_schedule(pspin_state* st, int groups);
 /* Tile shape of the pair-row kernel: waves per workgroup (1..16) and rows per
- * task (8 or 16, the loads in flight per lane).  pspin_autotune picks it. */
+ * batch (4 or 8; loads run one batch ahead per lane).  Fails with
+ * PSPIN_ERR_ARG when a wave would get more than 64 batches.  pspin_autotune
+ * picks it. */
 int pspin_set_pair_tiles(pspin_state* st, int nwaves, int rows);
 /* Coupling entries in flight per thread in the list kernel (8 or 16; 16 for
  * the dense variant only).  pspin_autotune picks it. */

@@ -140,8 +140,9 @@ def test_step_replay(pspin, oracle, name, variant, kernel, kw, T, nsw, fmt):
 
 @pytest.mark.parametrize("name,kernel,kw,groups,tiles", [
     ("dense FC N=64 pairs, 8 pair groups (bench system)", 2, dict(N=64, fmc_mode=0), 8, None),
-    ("dense FC N=64 pairs, 8 pair groups, 12 waves x 16 rows", 2, dict(N=64, fmc_mode=0), 8, (12, 16)),
-    ("dense FC N=40 pairs, per-replica pairs, 4 waves x 16 rows", 2, dict(N=40, fmc_mode=0), 0, (4, 16)),
+    ("dense FC N=64 pairs, 8 pair groups, 12 waves x 4 rows", 2, dict(N=64, fmc_mode=0), 8, (12, 4)),
+    ("dense FC N=40 pairs, per-replica pairs, 4 waves x 8 rows", 2, dict(N=40, fmc_mode=0), 0, (4, 8)),
+    ("dense FC N=64 pairs, 8 pair groups, 1 wave x 8 rows", 2, dict(N=64, fmc_mode=0), 8, (1, 8)),
     ("dense uniform N=32 lists, 4 pair groups", 1, dict(N=32, fmc_mode=2, gamma=0.05), 4, None),
     ("dense uniform N=64 lists, 16 entries in flight", 1, dict(N=64, fmc_mode=2, gamma=0.05), 0, 16),
 ])
