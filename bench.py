@@ -406,7 +406,7 @@ def main():
         "energy_ess_per_s_tmin": ess / wall,
         "energy_tau_int_tmin_sweeps": float(np.nanmean(taus)) if taus else None,
         "sweep_kernel_ms": kern_ms,
-        "exchange_kernel_ms": prof["exch_ms"] / max(prof["n_exch"], 1),
+        "exchange_kernel_ms": (prof["exch_ms"] / prof["n_exch"]) if prof["n_exch"] else None,
         "init_s": t_init,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": load_traffic(N, args.fmc, st.info.kernel),

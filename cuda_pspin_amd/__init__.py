@@ -25,6 +25,7 @@ LIB_PATH = os.environ.get("CUDA_PSPIN_AMD_LIB", os.path.join(_HERE, "libpspin.so
 
 DENSE = 0
 SPARSE = 1
+LAUNCH_AUTO, LAUNCH_GRAPHS, LAUNCH_DIRECT = 0, 1, 2
 KERNEL_AUTO, KERNEL_LISTS, KERNEL_PAIRS = 0, 1, 2
 
 STEP_DTYPE = np.dtype([("i0", "<i4"), ("j0", "<i4"), ("acc", "<i4"), ("dE", "<f4"),
@@ -132,6 +133,7 @@ def load_library() -> C.CDLL:
         "pspin_get_step_log": (C.c_int, [P, P, C.POINTER(C.c_int)]),
         "pspin_get_rng_info": (C.c_int, [P, u64p, C.POINTER(C.c_uint64)]),
         "pspin_set_profiling": (C.c_int, [P, C.c_int]),
+        "pspin_set_launch_mode": (C.c_int, [P, C.c_int]),
         "pspin_get_profile": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
         "pspin_set_kernel": (C.c_int, [P, C.c_int, C.c_int]),
@@ -353,6 +355,10 @@ class PSpin:
         c = C.c_uint64()
         _check(self._L.pspin_get_rng_info(self._h, s, C.byref(c)))
         return s, c.value
+
+    def set_launch_mode(self, mode: int) -> None:
+        """LAUNCH_AUTO (cooperative multi-round launches when they fit), LAUNCH_GRAPHS or LAUNCH_DIRECT."""
+        _check(self._L.pspin_set_launch_mode(self._h, int(mode)))
 
     def set_profiling(self, on: bool) -> None:
         _check(self._L.pspin_set_profiling(self._h, int(bool(on))))

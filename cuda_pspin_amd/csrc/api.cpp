@@ -378,7 +378,30 @@ int sweep_launch(pspin_state* st, int n, bool exchange = false) {
     if (st->prof) {
         PS_TRY_HIP(hipEventRecord(e1, st->stream));
         st->ev_sweep.push_back({e0, e1});
-        st->prof_n_sweep++;
+        st->prof_n_sweep += n;
+        if (st->ev_sweep.size() >= 512) {
+            int rc = harvest_events(st);
+            if (rc) return rc;
+        }
+    }
+    return PSPIN_OK;
+}
+
+// `rounds` PT rounds in one cooperative launch (timed like a sweep launch).
+int rounds_launch(pspin_state* st, int pt_freq, int rounds) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (st->prof) {
+        e0 = take_event(st);
+        e1 = take_event(st);
+        PS_TRY_HIP(hipEventRecord(e0, st->stream));
+    }
+    PS_TRY_HIP(ps::launch_pt_rounds(st, pt_freq, rounds));
+    const int n = rounds * pt_freq;
+    st->sweep_counter += (uint64_t)n;
+    if (st->prof) {
+        PS_TRY_HIP(hipEventRecord(e1, st->stream));
+        st->ev_sweep.push_back({e0, e1});
+        st->prof_n_sweep += n;
         if (st->ev_sweep.size() >= 512) {
             int rc = harvest_events(st);
             if (rc) return rc;
@@ -458,18 +481,25 @@ int pt_advance(pspin_state* st, int n_sweeps, int pt_freq) {
     PS_REQUIRE(st && n_sweeps >= 0 && pt_freq >= 1, PSPIN_ERR_ARG, "bad argument");
     PS_REQUIRE(st->pt_NT >= 2, PSPIN_ERR_STATE, "pspin_pt_init has not been called");
     int done = 0;
-    // Whole rounds (pt_freq sweeps + one exchange) go through a captured
-    // graph of kGraphRounds rounds when nothing per-launch is being recorded.
-    const bool can_graph = !st->prof && !st->d_log;
+    // Whole rounds (pt_freq sweeps + one exchange) go through one cooperative
+    // launch or a captured graph of `rounds` rounds (see pspin_set_launch_mode);
+    // nothing per sweep may be recorded, and graphs are not timed.
     const int rounds = std::max(1, kGraphSweeps / pt_freq);
     while (done < n_sweeps) {
         const int left = n_sweeps - done;
-        if (can_graph && st->pt_sweeps % pt_freq == 0 && left >= rounds * pt_freq) {
-            int rc = ensure_graph(st, pt_freq, rounds);
-            if (rc) return rc;
-            PS_TRY_HIP(hipGraphLaunch(st->graph, st->stream));
+        const bool whole = st->pt_sweeps % pt_freq == 0 && left >= rounds * pt_freq && !st->d_log;
+        const bool coop = whole && st->launch_mode == PSPIN_LAUNCH_AUTO && ps::pt_rounds_ok(st);
+        const bool graph = whole && !coop && !st->prof && st->launch_mode != PSPIN_LAUNCH_DIRECT;
+        if (coop || graph) {
+            int rc = PSPIN_OK;
+            if (coop) {
+                if ((rc = rounds_launch(st, pt_freq, rounds))) return rc;
+            } else {
+                if ((rc = ensure_graph(st, pt_freq, rounds))) return rc;
+                PS_TRY_HIP(hipGraphLaunch(st->graph, st->stream));
+                st->sweep_counter += (uint64_t)(rounds * pt_freq);
+            }
             const int nsw = rounds * pt_freq;
-            st->sweep_counter += (uint64_t)nsw;
             st->pt_sweeps += nsw;
             done += nsw;
             st->since_resync += nsw;
@@ -903,8 +933,8 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
     PS_REQUIRE(st && beta_sched, PSPIN_ERR_ARG, "null argument");
     PS_REQUIRE(NT >= 2 && nrep_per_temp >= 1 && (long long)NT * nrep_per_temp == st->nrep, PSPIN_ERR_ARG,
                "NT * nrep_per_temp must equal nrep and NT >= 2");
-    PS_REQUIRE((size_t)NT * nrep_per_temp * 12 + (size_t)NT * 16 <= 60 * 1024, PSPIN_ERR_ARG,
-               "too many replicas for the exchange kernel (NT * nrep_per_temp <= 5000)");
+    PS_REQUIRE(ps::exchange_lds_bytes(NT, nrep_per_temp) <= 60 * 1024, PSPIN_ERR_ARG,
+               "too many replicas for the exchange kernel (NT * nrep_per_temp <= 3000)");
     drop_graph(st);
     dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48); dfree(st->d_done);
     st->pt_NT = NT;
@@ -914,8 +944,8 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
     PS_TRY_HIP(dmalloc(&st->d_perm, (size_t)NT * nrep_per_temp));
     PS_TRY_HIP(dmalloc(&st->d_ex, (size_t)4 * (NT - 1)));
     PS_TRY_HIP(dmalloc(&st->d_rng48, 1));
-    PS_TRY_HIP(dmalloc(&st->d_done, 1));
-    PS_TRY_HIP(fill0(st, st->d_done, sizeof(unsigned int)));
+    PS_TRY_HIP(dmalloc(&st->d_done, 3));
+    PS_TRY_HIP(fill0(st, st->d_done, 3 * sizeof(unsigned int)));
     std::vector<int> perm((size_t)NT * nrep_per_temp);
     std::vector<double> betas((size_t)NT * nrep_per_temp);
     for (int i = 0; i < NT * nrep_per_temp; i++) {
@@ -935,6 +965,9 @@ int pspin_pt_run(pspin_state* st, int n_sweeps, int pt_freq) {
     const int rc = ps::pt_advance(st, n_sweeps, pt_freq);
     if (rc) return rc;
     PS_TRY_HIP(hipStreamSynchronize(st->stream));
+    unsigned int expired = 0;
+    PS_TRY_HIP(hipMemcpy(&expired, st->d_done + 2, sizeof(expired), hipMemcpyDeviceToHost));
+    PS_REQUIRE(!expired, PSPIN_ERR_HIP, "PT grid barrier wait expired: rounds ran without their exchange");
     return PSPIN_OK;
 }
 
@@ -1053,6 +1086,13 @@ int pspin_set_profiling(pspin_state* st, int on) {
     st->prof = on != 0;
     st->prof_sweep_ms = st->prof_exch_ms = 0.0;
     st->prof_n_sweep = st->prof_n_exch = 0;
+    return PSPIN_OK;
+}
+
+int pspin_set_launch_mode(pspin_state* st, int mode) {
+    PS_REQUIRE(st && mode >= PSPIN_LAUNCH_AUTO && mode <= PSPIN_LAUNCH_DIRECT, PSPIN_ERR_ARG,
+               "launch mode must be PSPIN_LAUNCH_AUTO, _GRAPHS or _DIRECT");
+    st->launch_mode = mode;
     return PSPIN_OK;
 }
 

@@ -6,122 +6,181 @@
 
 namespace ps {
 
-// Accept test u < exp(delta) for delta < 0, decided exactly as in FP64: an
-// FP32 exponential (relative error well under 1e-5 over the whole range,
-// flushing to 0 below about -87) settles every u outside a 2e-5 relative
-// band around it; inside the band the FP64 exp decides.
+// exp(x) for x <= 0 in FP64: Cody and Waite's reduction x = k ln2 + r (ln2
+// in two parts), then the Taylor series of e^r, |r| <= ln2/2, to 18 terms
+// (truncation below 1e-19 relative) by Horner's rule; a few ulp from the
+// correctly rounded value.  Only draws inside the accept band use it, so its
+// speed does not matter; it holds almost no constants, so none are hoisted
+// into the exchange loop (where they would be spilled).
+__device__ inline double exp_band(double x) {
+    if (x < -745.0) return 0.0;
+    const double k = rint(x * 1.4426950408889634);
+    const double r = fma(-k, 1.90821492927058770002e-10, fma(-k, 6.93147180369123816490e-01, x));
+    double p = 1.0;
+    for (int i = 18; i >= 1; i--) p = fma(p, r / (double)i, 1.0);
+    return ldexp(p, (int)k);
+}
+
+// Accept test u < exp(delta) for delta < 0, decided as in FP64: an FP32
+// exponential (relative error well under 1e-5 over the whole range, flushing
+// to 0 below about -87) settles every u outside a 2e-5 relative band around
+// it; inside the band the FP64 exponential decides.
 __device__ __forceinline__ bool accept_below(double u, double delta) {
     const double e32 = (double)__expf((float)delta);
     if (u < e32 * (1.0 - 2e-5)) return true;
     if (u > e32 * (1.0 + 2e-5)) return false;
-    return u < exp(delta);
+    return u < exp_band(delta);
 }
 
-// One wave, up to 64 ladders.  LDS: sE[n] energies by slot, sB[NT] betas,
-// sacc[NT] accepted swaps per level, sP[n] replicas by slot (n = NT nrep).
-// exchange_stage fills it with nthr threads (loads first, then LDS writes);
-// the caller then orders LDS (barrier or wave fence) before exchange_wave.
+// Exchange data crosses workgroups inside one launch (energies written by
+// every workgroup, the permutation and betas by the exchanging one), so it
+// moves with device-scope relaxed atomics: coherent across the XCDs' L2s
+// with no cache write-back or invalidation.  Ordering comes from the
+// arrival and release atomics after a workgroup fence.
+template <typename T>
+__device__ __forceinline__ T dev_load(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void dev_store(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// LDS of the one-wave exchange (n = NT nrep slots): sE[n] energies by slot,
+// sB[NT] betas, sacc[NT] accepted swaps per level, sX[n] the lcg48 states of
+// every draw the round can take, sP[n] replicas by slot.
+struct ExLds {
+    double* sE;
+    double* sB;
+    long long* sacc;
+    uint64_t* sX;
+    int* sP;
+    __device__ ExLds(void* lds, int NT, int n) {
+        sE = (double*)lds;
+        sB = sE + n;
+        sacc = (long long*)(sB + NT);
+        sX = (uint64_t*)(sacc + NT);
+        sP = (int*)(sX + n);
+    }
+};
+
+// exchange_stage fills the slots with nthr threads (loads first, then LDS
+// writes); the caller then orders LDS (barrier or wave fence) before
+// exchange_wave.
 __device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int nthr) {
+    asm volatile("" : "+v"(tid));  // nothing thread-derived hoisted out of a caller's loop
     const int NT = a.NT, n = NT * a.nrep;
-    double* sE = (double*)lds;
-    double* sB = sE + n;
-    long long* sacc = (long long*)(sB + NT);
-    int* sP = (int*)(sacc + NT);
+    ExLds L(lds, NT, n);
     for (int i0 = tid; i0 < n; i0 += 4 * nthr) {
         int p[4];
         double e[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) p[k] = i0 + k * nthr < n ? a.perm[i0 + k * nthr] : 0;
+        for (int k = 0; k < 4; k++) p[k] = i0 + k * nthr < n ? dev_load(a.perm + i0 + k * nthr) : 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) e[k] = i0 + k * nthr < n ? a.E[p[k]] : 0.0;
+        for (int k = 0; k < 4; k++) e[k] = i0 + k * nthr < n ? dev_load(a.E + p[k]) : 0.0;
 #pragma unroll
         for (int k = 0; k < 4; k++)
             if (i0 + k * nthr < n) {
-                sP[i0 + k * nthr] = p[k];
-                sE[i0 + k * nthr] = e[k];
+                L.sP[i0 + k * nthr] = p[k];
+                L.sE[i0 + k * nthr] = e[k];
             }
     }
-    for (int i = tid; i < NT; i += nthr) sB[i] = a.beta_sched[i];
+    for (int i = tid; i < NT; i += nthr) L.sB[i] = a.beta_sched[i];
 }
 
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The level chain: lane r carries ladder r's moving energy and replica; a
+// level's draws are read by rank from the precomputed states, so the chain
+// holds no permute, no 64-bit product and no global access.  The result goes
+// out after the last level.
 __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
+    // An opaque copy of the lane index: in a kernel that runs the exchange
+    // between sweeps, nothing lane-derived here may be hoisted out of that loop
+    // (it would stay live, and spilled, through every sweep).
+    asm volatile("" : "+v"(lane));
     const int NT = a.NT, nr = a.nrep, n = NT * nr;
     const bool valid = lane < nr;
-    double* sE = (double*)lds;
-    double* sB = sE + n;
-    long long* sacc = (long long*)(sB + NT);
-    int* sP = (int*)(sacc + NT);
-    // Draw of rank `lane` in a level: the lcg48 state X advanced lane + 1 steps.
-    uint64_t Ak, Ck;
-    lcg48_coeffs((uint64_t)lane + 1, &Ak, &Ck);
-    uint64_t X = *a.rng48;
+    ExLds L(lds, NT, n);
+    const uint64_t X0 = dev_load(a.rng48);
+    {
+        // X_{k+1} for k = 0 .. nr (NT - 1) - 1: lane l jumps l + 1 steps, then
+        // 64 steps at a time (lane 63's coefficients).
+        uint64_t A, C;
+        lcg48_coeffs((uint64_t)lane + 1, &A, &C);
+        const uint64_t A64 = readlane64(A, 63), C64 = readlane64(C, 63);
+        uint64_t x = (A * X0 + C) & kMask48;
+        for (int k = lane; k < nr * (NT - 1); k += 64) {
+            L.sX[k] = x;
+            x = (A64 * x + C64) & kMask48;
+        }
+    }
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int pc = valid ? sP[lane] : 0;         // replica carried down the column
-    double ec = valid ? sE[lane] : 0.0;    // its energy
-    double bt = sB[0];
-    int pn = valid ? sP[nr + lane] : 0;    // slot t+1 before level t
-    double en = valid ? sE[nr + lane] : 0.0;
-    double bn = sB[1];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int pc = valid ? L.sP[lane] : 0;         // replica carried down the column
+    double ec = valid ? L.sE[lane] : 0.0;    // its energy
+    double bt = L.sB[0];
+    int pn = valid ? L.sP[nr + lane] : 0;    // slot t+1 before level t
+    double en = valid ? L.sE[nr + lane] : 0.0;
+    double bn = L.sB[1];
+    int base = 0;  // draws taken so far
     for (int t = 0; t < NT - 1; t++) {
         const int pb = pn;
         const double eb = en, b1 = bn;
         if (t + 2 < NT) {  // next level's untouched slot, off the chain
-            pn = valid ? sP[(t + 2) * nr + lane] : 0;
-            en = valid ? sE[(t + 2) * nr + lane] : 0.0;
-            bn = sB[t + 2];
+            pn = valid ? L.sP[(t + 2) * nr + lane] : 0;
+            en = valid ? L.sE[(t + 2) * nr + lane] : 0.0;
+            bn = L.sB[t + 2];
         }
         const double delta = (bt - b1) * (ec - eb);
         const bool need = valid && !(delta >= 0.0);
         const unsigned long long m = __ballot(need);
-        const uint64_t x = (Ak * X + Ck) & kMask48;
-        const int rank = __popcll(m & lt);
-        const uint32_t xlo = (uint32_t)__shfl((int)(uint32_t)x, rank, 64);
-        const uint32_t xhi = (uint32_t)__shfl((int)(uint32_t)(x >> 32), rank, 64);
-        const int tot = __popcll(m);
-        if (tot > 0) {
-            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, tot - 1);
-            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), tot - 1);
-            X = ((uint64_t)hi << 32) | lo;
-        }
         bool accept = valid;
-        if (need) accept = accept_below(lcg48_unit(((uint64_t)xhi << 32) | xlo), delta);
-        // slot t keeps the carried replica unless the pair swaps
-        const int pf = accept ? pb : pc;
-        if (valid) {
-            a.perm[t * nr + lane] = pf;
-            a.betas[pf] = bt;
-        }
+        if (need) accept = accept_below(lcg48_unit(L.sX[base + __popcll(m & lt)]), delta);
+        base += __popcll(m);
+        // slot t keeps the carried replica unless the pair swaps (slot t is not
+        // read again: the prefetch is two slots ahead)
+        if (valid) L.sP[t * nr + lane] = accept ? pb : pc;
         if (!accept) {  // otherwise the carried replica moves down to slot t+1
             pc = pb;
             ec = eb;
         }
         bt = b1;
         const int nacc = __popcll(__ballot(valid && accept));
-        if (lane == 0) sacc[t] = nacc;
+        if (lane == 0) L.sacc[t] = nacc;
     }
-    if (valid) {
-        a.perm[(NT - 1) * nr + lane] = pc;
-        a.betas[pc] = bt;
-    }
+    if (valid) L.sP[(NT - 1) * nr + lane] = pc;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int i = lane; i < n; i += 64) {
+        const int p = L.sP[i];
+        dev_store(a.perm + i, p);
+        dev_store(a.betas + p, L.sB[i / nr]);
+    }
     for (int t = lane; t < NT - 1; t += 64) {  // all four loads before any store
-        const long long c0 = a.ex[0 * (NT - 1) + t], c1 = a.ex[1 * (NT - 1) + t];
-        const long long c2 = a.ex[2 * (NT - 1) + t], c3 = a.ex[3 * (NT - 1) + t];
-        const long long k = sacc[t];
-        a.ex[0 * (NT - 1) + t] = c0 + k;
-        a.ex[1 * (NT - 1) + t] = c1 + nr;
-        a.ex[2 * (NT - 1) + t] = c2 + k;
-        a.ex[3 * (NT - 1) + t] = c3 + nr;
+        long long* ex = a.ex;
+        const long long c0 = dev_load(ex + t), c1 = dev_load(ex + (NT - 1) + t);
+        const long long c2 = dev_load(ex + 2 * (NT - 1) + t), c3 = dev_load(ex + 3 * (NT - 1) + t);
+        const long long k = L.sacc[t];
+        dev_store(ex + t, c0 + k);
+        dev_store(ex + (NT - 1) + t, c1 + nr);
+        dev_store(ex + 2 * (NT - 1) + t, c2 + k);
+        dev_store(ex + 3 * (NT - 1) + t, c3 + nr);
     }
     if (a.series) {
-        const long long round = *a.series_n;
-        if (valid) a.series[(size_t)(round % a.series_cap) * nr + lane] = ec;
+        const long long round = dev_load(a.series_n);
+        if (valid) dev_store(a.series + (size_t)(round % a.series_cap) * nr + lane, ec);
     }
     if (lane == 0) {
-        *a.rng48 = X;
-        if (a.series) *a.series_n = *a.series_n + 1;
+        dev_store(a.rng48, base > 0 ? L.sX[base - 1] : X0);
+        if (a.series) dev_store(a.series_n, dev_load(a.series_n) + 1);
     }
 }
 

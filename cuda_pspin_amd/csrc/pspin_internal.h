@@ -236,7 +236,7 @@ struct pspin_state {
     double* d_series = nullptr;
     int series_cap = 0;
     long long* d_series_n = nullptr;
-    unsigned int* d_done = nullptr;  // workgroups finished in the current sweep launch (last one exchanges)
+    unsigned int* d_done = nullptr;  // [3]: workgroups arrived, barrier generation, barrier wait expired
 
     // step log
     ps::StepRec* d_log = nullptr;
@@ -250,6 +250,7 @@ struct pspin_state {
     bool prof = false;
     std::vector<hipEvent_t> ev_free;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_sweep, ev_exch;
+    int launch_mode = PSPIN_LAUNCH_AUTO;
     double prof_sweep_ms = 0.0, prof_exch_ms = 0.0;
     long long prof_n_sweep = 0, prof_n_exch = 0;
 };
@@ -290,7 +291,7 @@ struct ExArgs {
 ExArgs exchange_args(const pspin_state* st);
 // LDS bytes of the one-wave exchange for NT x nrep slots.
 inline size_t exchange_lds_bytes(int NT, int nrep) {
-    return (size_t)NT * nrep * (sizeof(double) + sizeof(int)) + (size_t)NT * 2 * sizeof(double);
+    return (size_t)NT * nrep * (2 * sizeof(double) + sizeof(int)) + (size_t)NT * 2 * sizeof(double);
 }
 // Whether a sweep launch can run the exchange in its last workgroup.
 bool exchange_fusable(const pspin_state* st);
@@ -298,6 +299,11 @@ bool exchange_fusable(const pspin_state* st);
 // Sweeps; with exchange = true the replica exchange follows (fused into the
 // same launch when exchange_fusable, else a separate launch).
 hipError_t launch_sweep(pspin_state* st, int n_sweeps, bool exchange = false);
+// `rounds` PT rounds (pt_freq sweeps, then the exchange) in one cooperative
+// launch with a grid barrier between rounds; pt_rounds_ok says whether every
+// workgroup can be resident at once (and the exchange can be fused).
+bool pt_rounds_ok(const pspin_state* st);
+hipError_t launch_pt_rounds(pspin_state* st, int pt_freq, int rounds);
 
 hipError_t launch_exchange(pspin_state* st);
 

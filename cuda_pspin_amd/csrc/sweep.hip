@@ -46,8 +46,12 @@
 
 namespace ps {
 
+// Grid-barrier wait bound (s_sleep 2 iterations, about a second).
+constexpr unsigned kBarrierSpinLimit = 1u << 23;
+
 struct SweepArgs {
     int N, nrep, h2_active, n_sweeps;
+    int rounds;             // PT rounds of n_sweeps sweeps + exchange (> 1: cooperative launch, grid barrier)
     int g2_lds;             // the FP32 two-body matrix is staged in dynamic LDS
     int pair_groups;        // 0: pairs from the replica stream; G > 0: from group (rep % G)'s stream
     uint64_t pair_seed;     // run seed the group keys derive from
@@ -71,7 +75,7 @@ struct SweepArgs {
     int npad;
     // replica exchange by the last workgroup to finish (exchange != 0)
     int exchange;
-    unsigned int* done;
+    unsigned int* done;     // [0] workgroups arrived, [1] barrier generation, [2] barrier wait expired
     ExArgs ex;
     const int4* tasks;        // row batches {slot | first << 1 | last << 2 | rows << 8, first pair, first row, tile offset}
     const int* wave_tasks;    // [nwaves+1] batch ranges per wave (at most 64 each)
@@ -336,6 +340,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     __shared__ float part[2][16];
     __shared__ Prop prop[3][2];  // [step mod 3][outcome of the step before]
     __shared__ int last_wg;
+    __shared__ unsigned int gen_seen;
     extern __shared__ float g2s[];  // [N][N] two-body couplings when g2_lds
 
     const int rep = blockIdx.x;
@@ -351,7 +356,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     // pad entries are read by zero-coupling rows of the pair tiles
     for (int i = lane; i < ((N + 3) & ~3); i += 64)
         s32[i] = i < N ? make_float2((float)gs[2 * i], (float)gs[2 * i + 1]) : make_float2(0.f, 0.f);
-    const double beta = a.betas[rep];
+    double beta = a.betas[rep];
     const uint64_t seed = a.seeds[rep];
     const uint64_t sweep0 = a.ctr[rep];
     const uint64_t gkey = a.pair_groups > 0 ? pair_group_key(a.pair_seed, rep % a.pair_groups) : 0;
@@ -371,14 +376,15 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         if (lane < nb) desc = a.tasks[t0 + lane];
     }
 
+    for (int rd = 0; rd < a.rounds; rd++) {
     for (int sw = 0; sw < a.n_sweeps; sw++) {
-        const uint64_t sweep = sweep0 + (uint64_t)sw;
+        const uint64_t sweep = sweep0 + (uint64_t)rd * a.n_sweeps + (uint64_t)sw;
         for (int k = tid; k < nsteps; k += bdim) pre[k] = make_step<CUBE>(seed, gkey, sweep, k, N, beta);
         __syncthreads();
         if (PAIRS && nb > 0) pair_issue<PB>(a, bA, desc_at(desc, 0), pre[0].i0, pre[0].j0, lane);
         StepRec* logp = nullptr;
-        if (a.log && a.log_base + sw < a.log_cap)
-            logp = a.log + ((size_t)rep * a.log_cap + a.log_base + sw) * nsteps;
+        const int slog = a.log_base + rd * a.n_sweeps + sw;
+        if (a.log && slog < a.log_cap) logp = a.log + ((size_t)rep * a.log_cap + slog) * nsteps;
         if (w == 0 && lane < 2) {
             const StepPre& P = pre[0];
             prop[0][lane] = propose<CUBE>(P, s64[P.i0], s64[P.j0]);
@@ -476,6 +482,52 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         }
         __syncthreads();  // pre[] and prop[] are rewritten by the next sweep
     }
+    if (a.exchange) {
+        // Last workgroup out runs the replica exchange on every energy.  Between
+        // rounds of a cooperative launch the others wait for it (a grid barrier:
+        // arrival count, then a generation word bumped after the exchange).
+        // The data crossing workgroups moves by device-scope atomics (see
+        // dev_load), so a workgroup fence (the store has landed) orders it.
+        const bool more = rd + 1 < a.rounds;
+        if (tid == 0) {
+            dev_store(a.E + rep, E);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            gen_seen = dev_load(a.done + 1);
+            last_wg = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+        }
+        __syncthreads();
+        if (last_wg) {
+            if (tid == 0) dev_store(a.done, 0u);
+            exchange_stage(a.ex, &view[0][0], tid, bdim);
+            __syncthreads();
+            if (w == 0) exchange_wave(a.ex, &view[0][0], lane);
+            if (more) {
+                if (w == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (tid == 0) __hip_atomic_fetch_add(a.done + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __syncthreads();
+                // the exchange staged its slots over the amplitude copies
+                for (int i = lane; i < ((N + 3) & ~3); i += 64)
+                    s32[i] = i < N ? make_float2((float)s64[i].x, (float)s64[i].y) : make_float2(0.f, 0.f);
+            }
+        } else if (more) {
+            if (tid == 0) {
+                // Bounded wait: every workgroup is resident (cooperative launch), so
+                // the release comes; past the bound the kernel flags it and goes on.
+                for (unsigned n = 0; dev_load(a.done + 1) == gen_seen; n++) {
+                    if (n == kBarrierSpinLimit) {
+                        __hip_atomic_fetch_or(a.done + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+        }
+        if (more) {
+            __syncthreads();
+            beta = dev_load(a.betas + rep);  // this replica's temperature after the exchange
+        }
+    }
+    }
 
     for (int i = tid; i < N; i += bdim) {
         gs[2 * i] = s64[i].x;
@@ -485,22 +537,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         a.E[rep] = E;
         a.acc[rep] += nacc;
         a.prop[rep] += nprop;
-        a.ctr[rep] = sweep0 + (uint64_t)a.n_sweeps;
-    }
-    if (a.exchange) {
-        // Last workgroup out runs the replica exchange on every energy.
-        if (tid == 0) {
-            __threadfence();
-            last_wg = atomicAdd(a.done, 1u) == gridDim.x - 1;
-        }
-        __syncthreads();
-        if (last_wg) {
-            __threadfence();
-            if (tid == 0) *a.done = 0u;
-            exchange_stage(a.ex, &view[0][0], tid, bdim);
-            __syncthreads();
-            if (w == 0) exchange_wave(a.ex, &view[0][0], lane);
-        }
+        a.ctr[rep] = sweep0 + (uint64_t)a.rounds * a.n_sweeps;
     }
 }
 
@@ -511,16 +548,37 @@ bool exchange_fusable(const pspin_state* st) {
            exchange_lds_bytes(st->pt_NT, st->pt_nrep) <= sizeof(float2) * 16 * kMaxN;
 }
 
-hipError_t launch_sweep(pspin_state* st, int n_sweeps, bool exchange) {
+using SweepKernel = void (*)(SweepArgs);
+
+// The kernel instance for the state's variant, layout, tile and coupling format.
+SweepKernel sweep_kernel(const pspin_state* st) {
+    const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
+    if (st->variant == PSPIN_SPARSE) return bf16 ? k_sweep<true, false, true> : k_sweep<true, false, false>;
+    if (st->kernel == PSPIN_KERNEL_PAIRS) return st->pair_rows == 4 ? k_sweep<false, true, false, 4> : k_sweep<false, true, false, 8>;
+    if (st->list_batch == 16) return bf16 ? k_sweep<false, false, true, 16> : k_sweep<false, false, false, 16>;
+    return bf16 ? k_sweep<false, false, true> : k_sweep<false, false, false>;
+}
+
+int sweep_block(const pspin_state* st) {
+    int bs = st->kernel == PSPIN_KERNEL_PAIRS ? 64 * st->pair_nwaves : st->block_size;
+    if (bs <= 0) {
+        // Enough threads that each touches a few entries per step.
+        long long per_step = 2LL * st->max_site_len;
+        bs = per_step >= 8192 ? 1024 : per_step >= 1024 ? 256 : 64;
+    }
+    return bs;
+}
+
+SweepArgs sweep_args(pspin_state* st, int n_sweeps, bool exchange, int rounds) {
     SweepArgs a;
     a.exchange = exchange && exchange_fusable(st);
     a.done = st->d_done;
     if (a.exchange) a.ex = exchange_args(st);
+    a.rounds = rounds;
     a.N = st->N;
     a.nrep = st->nrep;
     a.h2_active = st->h2_active;
     a.g2_lds = st->h2_active && (size_t)st->N * st->N * sizeof(float) <= kG2LdsMax;
-    const size_t shm = a.g2_lds ? (size_t)st->N * st->N * sizeof(float) : 0;
     a.n_sweeps = n_sweeps;
     a.spins = st->d_spins;
     a.g2 = st->d_g2_32;
@@ -543,33 +601,37 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps, bool exchange) {
     a.wave_tasks = st->d_wave_tasks;
     a.pair_groups = st->pair_groups;
     a.pair_seed = st->cfg.seed;
-    const bool pairs = st->kernel == PSPIN_KERNEL_PAIRS;
-    int bs = pairs ? 64 * st->pair_nwaves : st->block_size;
-    if (bs <= 0) {
-        // Enough threads that each touches a few entries per step.
-        long long per_step = 2LL * st->max_site_len;
-        bs = per_step >= 8192 ? 1024 : per_step >= 1024 ? 256 : 64;
-    }
-    const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
-    const bool lb16 = st->list_batch == 16;
-    if (st->variant == PSPIN_SPARSE) {
-        if (bf16) hipLaunchKernelGGL((k_sweep<true, false, true>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
-        else hipLaunchKernelGGL((k_sweep<true, false, false>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
-    } else if (pairs) {
-        if (st->pair_rows == 4)
-            hipLaunchKernelGGL((k_sweep<false, true, false, 4>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
-        else
-            hipLaunchKernelGGL((k_sweep<false, true, false, 8>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
-    } else if (lb16) {
-        if (bf16) hipLaunchKernelGGL((k_sweep<false, false, true, 16>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
-        else hipLaunchKernelGGL((k_sweep<false, false, false, 16>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
-    } else {
-        if (bf16) hipLaunchKernelGGL((k_sweep<false, false, true>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
-        else hipLaunchKernelGGL((k_sweep<false, false, false>), dim3(st->nrep), dim3(bs), shm, st->stream, a);
-    }
+    return a;
+}
+
+size_t sweep_shm(const SweepArgs& a) { return a.g2_lds ? (size_t)a.N * a.N * sizeof(float) : 0; }
+
+hipError_t launch_sweep(pspin_state* st, int n_sweeps, bool exchange) {
+    SweepArgs a = sweep_args(st, n_sweeps, exchange, 1);
+    hipLaunchKernelGGL(sweep_kernel(st), dim3(st->nrep), dim3(sweep_block(st)), sweep_shm(a), st->stream, a);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && exchange && !a.exchange) e = launch_exchange(st);
     return e;
+}
+
+bool pt_rounds_ok(const pspin_state* st) {
+    if (!exchange_fusable(st)) return false;
+    int dev = 0, coop = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return false;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
+    SweepArgs a = sweep_args(const_cast<pspin_state*>(st), 1, true, 1);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sweep_kernel(st), sweep_block(st),
+                                                     sweep_shm(a)) != hipSuccess)
+        return false;
+    return (long long)per_cu * ncu >= st->nrep;
+}
+
+hipError_t launch_pt_rounds(pspin_state* st, int pt_freq, int rounds) {
+    SweepArgs a = sweep_args(st, pt_freq, true, rounds);
+    void* args[] = {&a};
+    return hipLaunchCooperativeKernel((const void*)sweep_kernel(st), dim3(st->nrep), dim3(sweep_block(st)), args,
+                                      (unsigned)sweep_shm(a), st->stream);
 }
 
 }  // namespace ps

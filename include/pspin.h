@@ -212,11 +212,28 @@ int pspin_get_step_log(const pspin_state* st, void* out, int* n_sweeps_logged);
  * sweep counter that keys each sweep's Philox stream. */
 int pspin_get_rng_info(const pspin_state* st, uint64_t* seeds, uint64_t* sweep_counter);
 
-/* Sweep-kernel timing: when on, every sweep launch is bracketed by HIP
- * events on the library's stream; totals accumulate until reset. */
+/* Sweep-kernel timing: when on, every sweep launch (one or more sweeps, and
+ * the exchanges of a multi-round PT launch) is bracketed by HIP events on the
+ * library's stream; totals and the sweeps they cover accumulate until reset.
+ * Separate exchange launches are timed on their own.  Graph replays are not
+ * timed, so profiling never takes the graph path. */
 int pspin_set_profiling(pspin_state* st, int on);
-int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweep_launches, double* exch_ms,
+int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweeps, double* exch_ms,
                       long long* n_exch_launches);
+
+/* How pspin_pt_run launches whole PT rounds (pt_freq sweeps, one exchange).
+ * No counterpart in the reference (its host loop launches each sweep):
+ *   PSPIN_LAUNCH_AUTO    one cooperative launch of many rounds that meet at a
+ *                        grid barrier, the exchange run by the last workgroup
+ *                        to arrive, when every workgroup fits on the device at
+ *                        once; otherwise as PSPIN_LAUNCH_GRAPHS;
+ *   PSPIN_LAUNCH_GRAPHS  captured graphs of one sweep launch per round;
+ *   PSPIN_LAUNCH_DIRECT  one sweep launch per round from the host.
+ * The exchange runs in the last workgroup of the round's sweep launch when up
+ * to 64 ladders fit its LDS, else as a launch of its own.  Every mode gives
+ * the same chains bit for bit. */
+enum { PSPIN_LAUNCH_AUTO = 0, PSPIN_LAUNCH_GRAPHS = 1, PSPIN_LAUNCH_DIRECT = 2 };
+int pspin_set_launch_mode(pspin_state* st, int mode);
 
 /* cudaMemGetInfo / cudaGetDeviceProperties counterpart for the drivers'
  * memory pre-check and banner (src/parallel_tempering.cu:178-202).  dev < 0

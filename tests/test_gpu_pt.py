@@ -90,29 +90,43 @@ def test_pt_run_counts_and_betas(pspin, oracle):
     st.close()
 
 
-def test_graph_rounds_match_direct_launches(pspin, oracle):
-    # pt_run replays captured (sweep, exchange) rounds when nothing is being
-    # recorded per launch; with profiling on it launches directly.  Both must
-    # give the same chains: the kernels keep their RNG counters on the device.
-    NT, nr = 4, 2
+@pytest.mark.parametrize("N,NT,nr,kernel", [(16, 4, 2, 0), (64, 32, 8, 2)])
+def test_launch_modes_give_identical_chains(pspin, oracle, N, NT, nr, kernel):
+    # Whole PT rounds run as one cooperative launch with a grid barrier (auto),
+    # as captured graphs, or as direct launches (with the exchange in the
+    # sweep launch, or, when profiled, in a launch of its own).  The kernels
+    # keep their RNG counters and PT state on the device, so every way must
+    # give the same chains bit for bit.
     T, beta = oracle.temperatures(NT, 0.4, 1.6, 1)
     out = []
-    for prof in (False, True):
-        cfg = pspin.SimConfig.default(N=16, nrep=NT * nr, seed=321)
+    for mode, prof in ((pspin.LAUNCH_AUTO, False), (pspin.LAUNCH_GRAPHS, False), (pspin.LAUNCH_DIRECT, False),
+                       (pspin.LAUNCH_DIRECT, True), (pspin.LAUNCH_AUTO, True)):
+        print(f"[launch mode] {mode} profiling={prof}", flush=True)
+        cfg = pspin.SimConfig.default(N=N, nrep=NT * nr, seed=321)
         st = pspin.PSpin(cfg, 0)
+        if kernel:
+            st.set_kernel(kernel, 0)
+            st.set_pair_schedule(8)
         st.pt_init(NT, nr, beta, 321 + 5000)
+        st.set_launch_mode(mode)
         st.set_profiling(prof)
         st.pt_run(96, 1)
         st.pt_run(40, 3)
         E, acc, prop = st.results()
-        out.append((st.spins(), E, acc, st.pt_perm(), st.pt_rng48(), st.rng_info()[1]))
+        out.append((st.spins(), E, acc, st.pt_perm(), st.pt_rng48(), st.rng_info()[1], st.pt_exchange_stats()))
+        if prof:
+            p = st.profile()
+            assert p["n_sweep"] == 136
         st.close()
-    a, b = out
-    np.testing.assert_array_equal(a[0], b[0])
-    np.testing.assert_array_equal(a[1], b[1])
-    np.testing.assert_array_equal(a[2], b[2])
-    np.testing.assert_array_equal(a[3], b[3])
-    assert a[4] == b[4] and a[5] == b[5] == 136
+    a = out[0]
+    for b in out[1:]:
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+        np.testing.assert_array_equal(a[2], b[2])
+        np.testing.assert_array_equal(a[3], b[3])
+        for x, y in zip(a[6], b[6]):
+            np.testing.assert_array_equal(x, y)
+        assert a[4] == b[4] and a[5] == b[5] == 136
 
 
 def test_periodic_resync(pspin, oracle):
