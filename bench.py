@@ -376,7 +376,8 @@ def main():
 
     pu_per_step = nrep * (N // 2)
     value = args.steps * pu_per_step / wall
-    kern_ms = prof["sweep_ms"] / max(prof["n_sweep"], 1)
+    kern_ms = prof["sweep_ms"] / max(prof["n_sweep"], 1)  # per sweep (exchange included when fused)
+    sweeps_per_launch = prof["n_sweep"] / max(prof["n_launch"], 1)
     bpu = bytes_per_pair_update(N, st.info.h2_active)
     if args.fmc != 0:
         # FMC systems are stored sparsely: each surviving (site, term) entry is a
@@ -406,11 +407,14 @@ def main():
         "energy_ess_per_s_tmin": ess / wall,
         "energy_tau_int_tmin_sweeps": float(np.nanmean(taus)) if taus else None,
         "sweep_kernel_ms": kern_ms,
+        "sweeps_per_launch": sweeps_per_launch,
         "exchange_kernel_ms": (prof["exch_ms"] / prof["n_exch"]) if prof["n_exch"] else None,
         "init_s": t_init,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": load_traffic(N, args.fmc, st.info.kernel),
-                     "bytes_per_pair_update": bpu, "pair_updates_per_launch": pu_per_step},
+                     "bytes_per_pair_update": bpu,
+                     "pair_updates_per_launch": int(round(pu_per_step * sweeps_per_launch)),
+                     "launch_ms": kern_ms * sweeps_per_launch},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(st, N, nrep, np.repeat(beta, nr), args.seed, args.cpu_sweeps)

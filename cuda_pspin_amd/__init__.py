@@ -134,6 +134,7 @@ def load_library() -> C.CDLL:
         "pspin_get_rng_info": (C.c_int, [P, u64p, C.POINTER(C.c_uint64)]),
         "pspin_set_profiling": (C.c_int, [P, C.c_int]),
         "pspin_set_launch_mode": (C.c_int, [P, C.c_int]),
+        "pspin_get_profile_launches": (C.c_int, [P, C.POINTER(C.c_longlong)]),
         "pspin_get_profile": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
         "pspin_set_kernel": (C.c_int, [P, C.c_int, C.c_int]),
@@ -367,7 +368,10 @@ class PSpin:
         a, c = C.c_double(), C.c_double()
         b, d = C.c_longlong(), C.c_longlong()
         _check(self._L.pspin_get_profile(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
-        return {"sweep_ms": a.value, "n_sweep": b.value, "exch_ms": c.value, "n_exch": d.value}
+        e = C.c_longlong()
+        _check(self._L.pspin_get_profile_launches(self._h, C.byref(e)))
+        return {"sweep_ms": a.value, "n_sweep": b.value, "exch_ms": c.value, "n_exch": d.value,
+                "n_launch": e.value}
 
     def set_coupling_format(self, fmt: int) -> None:
         """0 = FP32, 1 = BF16 hi/lo pairs (list kernel only)."""

@@ -379,6 +379,7 @@ int sweep_launch(pspin_state* st, int n, bool exchange = false) {
         PS_TRY_HIP(hipEventRecord(e1, st->stream));
         st->ev_sweep.push_back({e0, e1});
         st->prof_n_sweep += n;
+        st->prof_n_launch++;
         if (st->ev_sweep.size() >= 512) {
             int rc = harvest_events(st);
             if (rc) return rc;
@@ -402,6 +403,7 @@ int rounds_launch(pspin_state* st, int pt_freq, int rounds) {
         PS_TRY_HIP(hipEventRecord(e1, st->stream));
         st->ev_sweep.push_back({e0, e1});
         st->prof_n_sweep += n;
+        st->prof_n_launch++;
         if (st->ev_sweep.size() >= 512) {
             int rc = harvest_events(st);
             if (rc) return rc;
@@ -1085,7 +1087,13 @@ int pspin_set_profiling(pspin_state* st, int on) {
     if (rc) return rc;
     st->prof = on != 0;
     st->prof_sweep_ms = st->prof_exch_ms = 0.0;
-    st->prof_n_sweep = st->prof_n_exch = 0;
+    st->prof_n_sweep = st->prof_n_exch = st->prof_n_launch = 0;
+    return PSPIN_OK;
+}
+
+int pspin_get_profile_launches(pspin_state* st, long long* n_sweep_launches) {
+    PS_REQUIRE(st && n_sweep_launches, PSPIN_ERR_ARG, "null argument");
+    *n_sweep_launches = st->prof_n_launch;
     return PSPIN_OK;
 }
 

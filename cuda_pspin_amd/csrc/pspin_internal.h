@@ -252,7 +252,7 @@ struct pspin_state {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_sweep, ev_exch;
     int launch_mode = PSPIN_LAUNCH_AUTO;
     double prof_sweep_ms = 0.0, prof_exch_ms = 0.0;
-    long long prof_n_sweep = 0, prof_n_exch = 0;
+    long long prof_n_sweep = 0, prof_n_exch = 0, prof_n_launch = 0;
 };
 
 // ---------------------------------------------------------------------------

@@ -220,6 +220,9 @@ int pspin_get_rng_info(const pspin_state* st, uint64_t* seeds, uint64_t* sweep_c
 int pspin_set_profiling(pspin_state* st, int on);
 int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweeps, double* exch_ms,
                       long long* n_exch_launches);
+/* Sweep launches behind pspin_get_profile's totals (a multi-round PT launch
+ * counts once).  No counterpart in the reference. */
+int pspin_get_profile_launches(pspin_state* st, long long* n_sweep_launches);
 
 /* How pspin_pt_run launches whole PT rounds (pt_freq sweeps, one exchange).
  * No counterpart in the reference (its host loop launches each sweep):
