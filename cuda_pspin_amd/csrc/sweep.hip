@@ -381,7 +381,9 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         const uint64_t sweep = sweep0 + (uint64_t)rd * a.n_sweeps + (uint64_t)sw;
         for (int k = tid; k < nsteps; k += bdim) pre[k] = make_step<CUBE>(seed, gkey, sweep, k, N, beta);
         __syncthreads();
-        if (PAIRS && nb > 0) pair_issue<PB>(a, bA, desc_at(desc, 0), pre[0].i0, pre[0].j0, lane);
+        if (PAIRS && nb > 0)
+            pair_issue<PB>(a, bA, desc_at(desc, 0), __builtin_amdgcn_readfirstlane(pre[0].i0),
+                           __builtin_amdgcn_readfirstlane(pre[0].j0), lane);
         StepRec* logp = nullptr;
         const int slog = a.log_base + rd * a.n_sweeps + sw;
         if (a.log && slog < a.log_cap) logp = a.log + ((size_t)rep * a.log_cap + slog) * nsteps;
@@ -427,7 +429,8 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
 
             const StepPre& P = pre[step];
             const Prop& Q = prop[step % 3][hyp];
-            const int i0 = P.i0, j0 = P.j0;
+            // wave-uniform: the address arithmetic of the tiles stays scalar
+            const int i0 = __builtin_amdgcn_readfirstlane(P.i0), j0 = __builtin_amdgcn_readfirstlane(P.j0);
             const float2 di = Q.di, dj = Q.dj;
             s32[i0] = Q.mi;
             s32[j0] = Q.mj;
@@ -455,14 +458,16 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             if (PAIRS) {
                 float2 acc[4];
                 uint4 inf = make_uint4(0u, 0u, 0u, 0u);
-                const int ni = step + 1 < nsteps ? pre[step + 1].i0 : 0, nj = step + 1 < nsteps ? pre[step + 1].j0 : 0;
+                const int ni = __builtin_amdgcn_readfirstlane(step + 1 < nsteps ? pre[step + 1].i0 : 0);
+                const int nj = __builtin_amdgcn_readfirstlane(step + 1 < nsteps ? pre[step + 1].j0 : 0);
+                int4 bd = nb > 0 ? desc_at(desc, 0) : make_int4(0, 0, 0, 0);
                 for (int b = 0; b < nb; b++) {
-                    const int4 bd = desc_at(desc, b);
                     int4 bn = make_int4(0, 0, 0, 0);
                     if (b + 1 < nb) bn = desc_at(desc, b + 1);
                     else if (step + 1 < nsteps) bn = desc_at(desc, 0);
                     const bool same = b + 1 < nb;
                     local += pair_step<PB>(a, bA, bd, bn, same ? i0 : ni, same ? j0 : nj, s32, di, dj, acc, inf, lane);
+                    bd = bn;
                 }
             } else
                 local += field_lists<BF16, PB>(a, s32, i0, j0, di, dj, tid, bdim);
