@@ -121,6 +121,7 @@ def learned_vs_plain(args):
         st.set_profiling(True)  # per-launch events instead of captured graphs
     st.pt_init(NT, nr, 1.0 / T, args.seed + 5000)
     st.pt_run(args.lp_burn, 1)
+    tuned = st.autotune(8)  # the sweep kernel both runs use; restores the state
     lcfg = P.LPConfig.default(hidden=512, batch_cap=4096, data_cap=args.lp_snapshots, seed=args.seed)
     lp = P.LearnedMove(st, lcfg)
     t0 = time.perf_counter()
@@ -179,7 +180,8 @@ def learned_vs_plain(args):
                                  f"[{args.Tmin},{args.Tmax}] geometric, exchange every sweep",
                      "learned": f"one move per {args.lp_every} sweeps, scale {args.lp_scale}, mean scale "
                                 f"{args.lp_scale ** 2:.3g}", "blocks": args.lp_blocks, "sweeps_per_block": S,
-                     "burn_in_sweeps": args.lp_burn}
+                     "burn_in_sweeps": args.lp_burn,
+                     "sweep_kernel": {"kernel": tuned[0], "block": tuned[1], "trial_ms_per_sweep": tuned[2]}}
     return out
 
 

@@ -26,7 +26,7 @@ LIB_PATH = os.environ.get("CUDA_PSPIN_AMD_LIB", os.path.join(_HERE, "libpspin.so
 DENSE = 0
 SPARSE = 1
 LAUNCH_AUTO, LAUNCH_GRAPHS, LAUNCH_DIRECT = 0, 1, 2
-KERNEL_AUTO, KERNEL_LISTS, KERNEL_PAIRS = 0, 1, 2
+KERNEL_AUTO, KERNEL_LISTS, KERNEL_PAIRS, KERNEL_GROUPS = 0, 1, 2, 3
 
 STEP_DTYPE = np.dtype([("i0", "<i4"), ("j0", "<i4"), ("acc", "<i4"), ("dE", "<f4"),
                        ("ai", "<f8", (2,)), ("aj", "<f8", (2,))])

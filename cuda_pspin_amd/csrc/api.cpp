@@ -66,6 +66,7 @@ hipError_t fill0(const pspin_state* st, void* dst, size_t bytes) {
 void free_all(pspin_state* st) {
     dfree(st->d_g2_64); dfree(st->d_g2_32); dfree(st->d_g4_64); dfree(st->d_mask);
     dfree(st->d_sq); dfree(st->d_sqg); dfree(st->d_site_off); dfree(st->d_site_ent);
+    dfree(st->d_grp_off); dfree(st->d_grp_hdr); dfree(st->d_grp_g);
     dfree(st->d_spins); dfree(st->d_E); dfree(st->d_betas); dfree(st->d_acc); dfree(st->d_prop);
     dfree(st->d_seeds); dfree(st->d_e2); dfree(st->d_e4); dfree(st->d_ctr);
     if (st->graph) (void)hipGraphExecDestroy(st->graph);
@@ -132,6 +133,64 @@ int build_site_lists(pspin_state* st, const std::vector<Term>& terms) {
     PS_TRY_HIP(cpy(st, st->d_site_off, cnt.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
     if (!ent.empty())
         PS_TRY_HIP(cpy(st, st->d_site_ent, ent.data(), ent.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    return PSPIN_OK;
+}
+
+// Per-site chunks of the group kernel.  Site x's entries (p, m, n, g) are
+// grouped by the conjugated pair {m, n}: the field of x gets
+// conj(a_m a_n) * sum_p g a_p from each group, one product per chunk of up
+// to four partners.  Groups are emitted in (m, n) order and cut into chunks
+// of 4; unused slots have g = 0 and p = m.
+int build_site_groups(pspin_state* st, const std::vector<Term>& terms) {
+    const int N = st->N;
+    std::vector<std::vector<std::pair<uint32_t, std::pair<int, float>>>> per(N);  // key m | n << 8, (p, g)
+    for (const Term& t : terms) {
+        const float g = (float)t.g;
+        for (int s = 0; s < 4; s++) {
+            const uint32_t e = pack_entry(t.v, t.ch, s);
+            const int p = e & 0xff, m = (e >> 8) & 0xff, n = (e >> 16) & 0xff;
+            const uint32_t key = (uint32_t)std::min(m, n) | ((uint32_t)std::max(m, n) << 8);
+            per[t.v[s]].push_back({key, {p, g}});
+        }
+    }
+    std::vector<int> off(N + 1, 0);
+    std::vector<uint2> hdr;
+    std::vector<uint4> gw;
+    for (int x = 0; x < N; x++) {
+        auto& v = per[x];
+        std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (size_t i = 0; i < v.size();) {
+            size_t j = i;
+            while (j < v.size() && v[j].first == v[i].first) j++;
+            for (size_t c = i; c < j; c += 4) {
+                const uint32_t m = v[i].first & 0xff;
+                uint32_t ps_[4] = {m, m, m, m}, gb[4] = {0u, 0u, 0u, 0u};
+                for (size_t k = c; k < std::min(j, c + 4); k++) {
+                    ps_[k - c] = (uint32_t)v[k].second.first;
+                    std::memcpy(&gb[k - c], &v[k].second.second, 4);
+                }
+                hdr.push_back(make_uint2(v[i].first, ps_[0] | ps_[1] << 8 | ps_[2] << 16 | ps_[3] << 24));
+                gw.push_back(make_uint4(gb[0], gb[1], gb[2], gb[3]));
+            }
+            i = j;
+        }
+        off[x + 1] = (int)hdr.size();
+        v.clear();
+        v.shrink_to_fit();
+    }
+    st->n_grp = (long long)hdr.size();
+    st->max_site_grp = 0;
+    for (int x = 0; x < N; x++) st->max_site_grp = std::max(st->max_site_grp, off[x + 1] - off[x]);
+    // one all-zero chunk after the last: the sweep's loads past a range read it
+    hdr.push_back(make_uint2(0u, 0u));
+    gw.push_back(make_uint4(0u, 0u, 0u, 0u));
+    st->h_grp_g = gw;
+    PS_TRY_HIP(dmalloc(&st->d_grp_off, N + 1));
+    PS_TRY_HIP(dmalloc(&st->d_grp_hdr, hdr.size()));
+    PS_TRY_HIP(dmalloc(&st->d_grp_g, gw.size()));
+    PS_TRY_HIP(cpy(st, st->d_grp_off, off.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
+    PS_TRY_HIP(cpy(st, st->d_grp_hdr, hdr.data(), hdr.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    PS_TRY_HIP(cpy(st, st->d_grp_g, gw.data(), gw.size() * sizeof(uint4), hipMemcpyHostToDevice));
     return PSPIN_OK;
 }
 
@@ -750,6 +809,10 @@ int pspin_create(const pspin_sim_config* cfg, int variant, pspin_state** out) {
     }
     (void)hipFree(d_omega);
     if ((rc = build_site_lists(st, terms))) return fail(rc);
+    if (variant == PSPIN_DENSE) {
+        if ((rc = build_site_groups(st, terms))) return fail(rc);
+        if (st->kernel != PSPIN_KERNEL_PAIRS) st->kernel = PSPIN_KERNEL_GROUPS;
+    }
     terms.clear();
     terms.shrink_to_fit();
 
@@ -1115,6 +1178,10 @@ int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweep, dou
     return PSPIN_OK;
 }
 
+static const char* kernel_name(int k) {
+    return k == PSPIN_KERNEL_PAIRS ? "pair-row" : k == PSPIN_KERNEL_GROUPS ? "group" : "list";
+}
+
 // BF16 hi/lo split of an FP32 coupling: hi = bf16(g) (round to nearest even),
 // lo = bf16(g - hi); packed hi | lo << 16.
 static uint32_t bf16_rne(float f) {
@@ -1127,22 +1194,34 @@ static uint32_t bf16_rne(float f) {
 int pspin_set_coupling_format(pspin_state* st, int format) {
     PS_REQUIRE(st, PSPIN_ERR_ARG, "null argument");
     PS_REQUIRE(format == PSPIN_COUPLING_F32 || format == PSPIN_COUPLING_BF16X2, PSPIN_ERR_ARG, "unknown format");
-    PS_REQUIRE(format == PSPIN_COUPLING_F32 || st->kernel == PSPIN_KERNEL_LISTS, PSPIN_ERR_STATE,
-               "BF16 hi/lo couplings are served by the list kernel (pspin_set_kernel(.., PSPIN_KERNEL_LISTS, ..))");
+    PS_REQUIRE(format == PSPIN_COUPLING_F32 || st->kernel == PSPIN_KERNEL_LISTS || st->kernel == PSPIN_KERNEL_GROUPS,
+               PSPIN_ERR_STATE,
+               "BF16 hi/lo couplings are served by the list and group kernels (pspin_set_kernel(.., "
+               "PSPIN_KERNEL_LISTS or PSPIN_KERNEL_GROUPS, ..))");
+    auto enc = [format](uint32_t& w) {
+        if (format != PSPIN_COUPLING_BF16X2) return;
+        float g;
+        std::memcpy(&g, &w, 4);
+        const uint32_t hi = bf16_rne(g);
+        float hf;
+        const uint32_t hb = hi << 16;
+        std::memcpy(&hf, &hb, 4);
+        const uint32_t lo = bf16_rne(g - hf);
+        w = hi | (lo << 16);
+    };
     std::vector<uint2> ent = st->h_site_ent;
-    if (format == PSPIN_COUPLING_BF16X2)
-        for (uint2& e : ent) {
-            float g;
-            std::memcpy(&g, &e.y, 4);
-            const uint32_t hi = bf16_rne(g);
-            float hf;
-            const uint32_t hb = hi << 16;
-            std::memcpy(&hf, &hb, 4);
-            const uint32_t lo = bf16_rne(g - hf);
-            e.y = hi | (lo << 16);
-        }
+    for (uint2& e : ent) enc(e.y);
     if (!ent.empty())
         PS_TRY_HIP(cpy(st, st->d_site_ent, ent.data(), ent.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    std::vector<uint4> gw = st->h_grp_g;
+    for (uint4& g : gw) {
+        enc(g.x);
+        enc(g.y);
+        enc(g.z);
+        enc(g.w);
+    }
+    if (!gw.empty())
+        PS_TRY_HIP(cpy(st, st->d_grp_g, gw.data(), gw.size() * sizeof(uint4), hipMemcpyHostToDevice));
     drop_graph(st);
     st->coupling_format = format;
     return PSPIN_OK;
@@ -1223,6 +1302,8 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
     for (int lb : {8, 16}) {
         if (lb == 16 && st->variant != PSPIN_DENSE) continue;
         for (int bs : {64, 128, 256, 512, 1024}) cands.push_back({PSPIN_KERNEL_LISTS, bs, lb});
+        if (st->d_grp_off)
+            for (int bs : {128, 256, 512, 1024}) cands.push_back({PSPIN_KERNEL_GROUPS, bs, lb});
     }
     const int old_kernel = st->kernel, old_block = st->block_size, old_nw = st->pair_nwaves,
               old_rows = st->pair_rows, old_lb = st->list_batch;
@@ -1262,7 +1343,7 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
         const double per = ms / trial_sweeps;
         if (verbose)
             std::printf("[auto-tune] sweep N=%d nrep=%d %s %s=%d depth=%d: %.4f ms/sweep\n", st->N, st->nrep,
-                        c.kernel == PSPIN_KERNEL_PAIRS ? "pair-row" : "list", c.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block",
+                        kernel_name(c.kernel), c.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block",
                         c.param, c.rows, per);
         if (per < best) {
             best = per;
@@ -1293,7 +1374,7 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
     if (rc) return rc;
     if (verbose)
         std::printf("[auto-tune] sweep N=%d -> %s %s=%d depth=%d (%.4f ms/sweep)\n", st->N,
-                    pick.kernel == PSPIN_KERNEL_PAIRS ? "pair-row" : "list",
+                    kernel_name(pick.kernel),
                     pick.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block", pick.param, pick.rows, best);
     if (kernel) *kernel = pick.kernel;
     if (block) *block = pick.kernel == PSPIN_KERNEL_PAIRS ? 64 * pick.param : pick.param;
@@ -1333,14 +1414,20 @@ int pspin_get_pair_keys(const pspin_state* st, uint64_t* keys) {
 
 int pspin_set_kernel(pspin_state* st, int kernel, int block_size) {
     PS_REQUIRE(st, PSPIN_ERR_ARG, "null argument");
-    PS_REQUIRE(kernel == PSPIN_KERNEL_AUTO || kernel == PSPIN_KERNEL_LISTS || kernel == PSPIN_KERNEL_PAIRS,
+    PS_REQUIRE(kernel == PSPIN_KERNEL_AUTO || kernel == PSPIN_KERNEL_LISTS || kernel == PSPIN_KERNEL_PAIRS ||
+                   kernel == PSPIN_KERNEL_GROUPS,
                PSPIN_ERR_ARG, "unknown kernel");
     PS_REQUIRE(kernel != PSPIN_KERNEL_PAIRS || st->d_ptab, PSPIN_ERR_ARG,
                "PSPIN_KERNEL_PAIRS needs a dense fully connected system (fmc_mode 0, 8 <= N <= 128)");
+    PS_REQUIRE(kernel != PSPIN_KERNEL_GROUPS || st->d_grp_off, PSPIN_ERR_ARG,
+               "PSPIN_KERNEL_GROUPS needs the dense variant");
     PS_REQUIRE(block_size == 0 || (block_size % 64 == 0 && block_size >= 64 && block_size <= 1024), PSPIN_ERR_ARG,
                "block_size must be 0 or a multiple of 64 in [64, 1024]");
-    const int k = kernel == PSPIN_KERNEL_AUTO ? (st->d_ptab ? PSPIN_KERNEL_PAIRS : PSPIN_KERNEL_LISTS) : kernel;
-    PS_REQUIRE(k == PSPIN_KERNEL_LISTS || st->coupling_format == PSPIN_COUPLING_F32, PSPIN_ERR_STATE,
+    const int k = kernel != PSPIN_KERNEL_AUTO ? kernel
+                  : st->d_ptab                ? PSPIN_KERNEL_PAIRS
+                  : st->d_grp_off             ? PSPIN_KERNEL_GROUPS
+                                              : PSPIN_KERNEL_LISTS;
+    PS_REQUIRE(k != PSPIN_KERNEL_PAIRS || st->coupling_format == PSPIN_COUPLING_F32, PSPIN_ERR_STATE,
                "the pair-row kernel stores FP32 couplings; set PSPIN_COUPLING_F32 first");
     drop_graph(st);
     st->kernel = k;

@@ -192,6 +192,15 @@ struct pspin_state {
     int max_site_len = 0;
     std::vector<uint2> h_site_ent;  // FP32 entries, kept to re-encode the coupling words
     int coupling_format = PSPIN_COUPLING_F32;
+    // dense: per-site chunks of H4 entries sharing the conjugated pair (m, n)
+    // (PSPIN_KERNEL_GROUPS): header {m | n << 8, p0 | p1 << 8 | p2 << 16 | p3 << 24}
+    // and four coupling words (FP32 bits or BF16 hi/lo), zero in unused slots
+    int* d_grp_off = nullptr;       // [N+1] chunk offsets per site
+    uint2* d_grp_hdr = nullptr;     // [n_grp + 1], the last chunk all zero
+    uint4* d_grp_g = nullptr;       // [n_grp + 1]
+    long long n_grp = 0;
+    int max_site_grp = 0;
+    std::vector<uint4> h_grp_g;     // FP32 words, kept to re-encode the coupling words
 
     // dense fully connected per-site pair tiles (PSPIN_KERNEL_PAIRS)
     float* d_ptab = nullptr;       // [N][pair_slice_len]

@@ -79,7 +79,15 @@ struct SweepArgs {
     ExArgs ex;
     const int4* tasks;        // row batches {slot | first << 1 | last << 2 | rows << 8, first pair, first row, tile offset}
     const int* wave_tasks;    // [nwaves+1] batch ranges per wave (at most 64 each)
+    // GROUPS layout
+    const int* grp_off;       // [N+1] chunk offsets per site
+    const uint2* grp_hdr;     // [chunks] {m | n << 8, p0 | p1 << 8 | p2 << 16 | p3 << 24}
+    const uint4* grp_g;       // [chunks] four coupling words
+    int grp_zero;             // index of the all-zero chunk after the last
 };
+
+// Coupling layouts of the sweep kernel.
+constexpr int kLists = 0, kPairs = 1, kGroups = 2;
 
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -283,6 +291,66 @@ __device__ __forceinline__ float pair_step(const SweepArgs& a, PairBuf<PB>& B, i
     return -(d.x * Hc.x + d.y * Hc.y);
 }
 
+// Local-field contribution of the thread, GROUPS layout (dense, any channel
+// pattern).  Site x's H4 entries are grouped by their conjugated pair {m, n}
+// and cut into chunks of up to four partners p, so a chunk adds
+// conj(s~_m s~_n) * sum_k g_k s~_pk to the field of x: 3 LDS reads and 4 FMA
+// per chunk plus 1 LDS read and 2 FMA per entry, against 3 reads and about
+// 10 FMA per entry in the LISTS layout.  The chunk ranges of the step's two
+// sites are laid end to end; a thread takes chunks tid, tid + bdim, ... GB at
+// a time.  Loads run one batch ahead of the arithmetic, and the last batch of
+// a step loads the next step's first batch, so those loads cross the barrier.
+template <int GB>
+struct GrpBuf {
+    uint2 h[GB];
+    uint4 g[GB];
+};
+
+struct GrpRange {
+    int oi, ci, oj, n_tot;  // chunk offset and count of site i0, offset of j0, total
+};
+
+// Wave-uniform (the sites are), so the batch loop runs on the scalar unit.
+__device__ __forceinline__ GrpRange grp_range(const int* goff, int i0, int j0) {
+    GrpRange r;
+    r.oi = __builtin_amdgcn_readfirstlane(goff[i0]);
+    r.ci = __builtin_amdgcn_readfirstlane(goff[i0 + 1]) - r.oi;
+    r.oj = __builtin_amdgcn_readfirstlane(goff[j0]);
+    r.n_tot = r.ci + __builtin_amdgcn_readfirstlane(goff[j0 + 1]) - r.oj;
+    return r;
+}
+
+// Loads chunk e of range r into slot u of the buffer; past the range, the
+// all-zero chunk (s~_0 with zero couplings).  No branch, so the buffer keeps
+// its registers from one batch to the next.
+template <int GB>
+__device__ __forceinline__ void grp_load(const SweepArgs& a, GrpBuf<GB>& B, int u, const GrpRange& r, int e) {
+    const int idx = e < r.ci ? r.oi + e : (e < r.n_tot ? r.oj + (e - r.ci) : a.grp_zero);
+    B.h[u] = a.grp_hdr[idx];
+    B.g[u] = a.grp_g[idx];
+}
+
+// The chunk in slot u: -Re[d conj(s~_m s~_n) sum_k g_k s~_pk], d of its site.
+template <bool BF16, int GB>
+__device__ __forceinline__ float grp_chunk(const GrpBuf<GB>& B, int u, const float2* s32, bool isj, float2 di,
+                                           float2 dj) {
+    const uint2 h = B.h[u];
+    const float2 sm = s32[h.x & 0xff], sn = s32[(h.x >> 8) & 0xff];
+    const float cx = sm.x * sn.x - sm.y * sn.y, cy = -(sm.x * sn.y + sm.y * sn.x);  // conj(s_m s_n)
+    const uint32_t gw[4] = {B.g[u].x, B.g[u].y, B.g[u].z, B.g[u].w};
+    float wx = 0.f, wy = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float2 sp = s32[(h.y >> (8 * k)) & 0xff];
+        const float g = coupling<BF16>(gw[k]);
+        wx += g * sp.x;
+        wy += g * sp.y;
+    }
+    const float xr = wx * cx - wy * cy, xi = wx * cy + wy * cx;
+    const float2 d = isj ? dj : di;
+    return -(d.x * xr - d.y * xi);
+}
+
 // A proposal for one step under one hypothesis about the step before it.
 struct Prop {
     double2 ni, nj;      // proposed FP64 amplitudes
@@ -332,8 +400,9 @@ __device__ __forceinline__ Prop propose(const StepPre& P, double2 ai, double2 aj
 // midpoints of step t+1's.  Private copies are written with wave-uniform
 // addresses and values by all lanes, so each wave's LDS program order is the
 // only ordering needed.  Wave 0 owns the FP64 amplitudes.
-template <bool CUBE, bool PAIRS, bool BF16, int PB = 8>
+template <bool CUBE, int LAYOUT, bool BF16, int PB = 8>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
+    constexpr bool PAIRS = LAYOUT == kPairs, GROUPS = LAYOUT == kGroups;
     __shared__ double2 s64[kMaxN];
     __shared__ __align__(16) float2 view[16][kMaxN];
     __shared__ StepPre pre[kMaxN / 2];
@@ -341,6 +410,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     __shared__ Prop prop[3][2];  // [step mod 3][outcome of the step before]
     __shared__ int last_wg;
     __shared__ unsigned int gen_seen;
+    __shared__ int goff[kMaxN + 1];  // GROUPS: chunk offsets per site
     extern __shared__ float g2s[];  // [N][N] two-body couplings when g2_lds
 
     const int rep = blockIdx.x;
@@ -365,6 +435,9 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     if (tid == 0) E = a.E[rep];
     if (a.h2_active && a.g2_lds)
         for (int i = tid; i < N * N; i += bdim) g2s[i] = a.g2[i];
+    if (GROUPS)
+        for (int i = tid; i <= N; i += bdim) goff[i] = a.grp_off[i];
+    GrpBuf<PB> gB;
 
     // PAIRS: this wave's batch descriptors, one per lane (an even count)
     int4 desc = make_int4(0, 0, 0, 0);
@@ -384,6 +457,12 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         if (PAIRS && nb > 0)
             pair_issue<PB>(a, bA, desc_at(desc, 0), __builtin_amdgcn_readfirstlane(pre[0].i0),
                            __builtin_amdgcn_readfirstlane(pre[0].j0), lane);
+        if (GROUPS) {
+            const GrpRange r0 = grp_range(goff, __builtin_amdgcn_readfirstlane(pre[0].i0),
+                                          __builtin_amdgcn_readfirstlane(pre[0].j0));
+#pragma unroll
+            for (int u = 0; u < PB; u++) grp_load<PB>(a, gB, u, r0, tid + u * bdim);
+        }
         StepRec* logp = nullptr;
         const int slog = a.log_base + rd * a.n_sweeps + sw;
         if (a.log && slog < a.log_cap) logp = a.log + ((size_t)rep * a.log_cap + slog) * nsteps;
@@ -468,6 +547,26 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                     const bool same = b + 1 < nb;
                     local += pair_step<PB>(a, bA, bd, bn, same ? i0 : ni, same ? j0 : nj, s32, di, dj, acc, inf, lane);
                     bd = bn;
+                }
+            } else if (GROUPS) {
+                const GrpRange r = grp_range(goff, i0, j0);
+                const bool more = step + 1 < nsteps;
+                const int ni = __builtin_amdgcn_readfirstlane(more ? pre[step + 1].i0 : 0);
+                const int nj = __builtin_amdgcn_readfirstlane(more ? pre[step + 1].j0 : 1);
+                const GrpRange rn = grp_range(goff, ni, nj);
+                const int per = PB * bdim;
+                const int nbt = r.n_tot > per ? (r.n_tot + per - 1) / per : 1;
+                // As soon as a chunk has been used, its registers take the
+                // same slot of the next batch (of this step or the next).
+                for (int b = 0; b < nbt; b++) {
+                    const bool same = b + 1 < nbt;
+                    const GrpRange& rl = same ? r : rn;
+                    const int e1 = same ? tid + (b + 1) * per : tid;
+#pragma unroll
+                    for (int u = 0; u < PB; u++) {
+                        local += grp_chunk<BF16, PB>(gB, u, s32, tid + b * per + u * bdim >= r.ci, di, dj);
+                        grp_load<PB>(a, gB, u, rl, e1 + u * bdim);  // (after a sweep's last step: unused)
+                    }
                 }
             } else
                 local += field_lists<BF16, PB>(a, s32, i0, j0, di, dj, tid, bdim);
@@ -558,15 +657,24 @@ using SweepKernel = void (*)(SweepArgs);
 // The kernel instance for the state's variant, layout, tile and coupling format.
 SweepKernel sweep_kernel(const pspin_state* st) {
     const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
-    if (st->variant == PSPIN_SPARSE) return bf16 ? k_sweep<true, false, true> : k_sweep<true, false, false>;
-    if (st->kernel == PSPIN_KERNEL_PAIRS) return st->pair_rows == 4 ? k_sweep<false, true, false, 4> : k_sweep<false, true, false, 8>;
-    if (st->list_batch == 16) return bf16 ? k_sweep<false, false, true, 16> : k_sweep<false, false, false, 16>;
-    return bf16 ? k_sweep<false, false, true> : k_sweep<false, false, false>;
+    if (st->variant == PSPIN_SPARSE) return bf16 ? k_sweep<true, kLists, true> : k_sweep<true, kLists, false>;
+    if (st->kernel == PSPIN_KERNEL_PAIRS)
+        return st->pair_rows == 4 ? k_sweep<false, kPairs, false, 4> : k_sweep<false, kPairs, false, 8>;
+    if (st->kernel == PSPIN_KERNEL_GROUPS) {
+        if (st->list_batch == 16) return bf16 ? k_sweep<false, kGroups, true, 6> : k_sweep<false, kGroups, false, 6>;
+        return bf16 ? k_sweep<false, kGroups, true, 4> : k_sweep<false, kGroups, false, 4>;
+    }
+    if (st->list_batch == 16) return bf16 ? k_sweep<false, kLists, true, 16> : k_sweep<false, kLists, false, 16>;
+    return bf16 ? k_sweep<false, kLists, true> : k_sweep<false, kLists, false>;
 }
 
 int sweep_block(const pspin_state* st) {
     int bs = st->kernel == PSPIN_KERNEL_PAIRS ? 64 * st->pair_nwaves : st->block_size;
-    if (bs <= 0) {
+    if (bs <= 0 && st->kernel == PSPIN_KERNEL_GROUPS) {
+        // A few chunks per thread per step.
+        const long long per_step = 2LL * st->max_site_grp;
+        bs = per_step >= 4096 ? 1024 : per_step >= 2048 ? 512 : per_step >= 512 ? 256 : 64;
+    } else if (bs <= 0) {
         // Enough threads that each touches a few entries per step.
         long long per_step = 2LL * st->max_site_len;
         bs = per_step >= 8192 ? 1024 : per_step >= 1024 ? 256 : 64;
@@ -604,6 +712,10 @@ SweepArgs sweep_args(pspin_state* st, int n_sweeps, bool exchange, int rounds) {
     a.npad = st->pair_npad;
     a.tasks = st->d_tasks;
     a.wave_tasks = st->d_wave_tasks;
+    a.grp_off = st->d_grp_off;
+    a.grp_hdr = st->d_grp_hdr;
+    a.grp_g = st->d_grp_g;
+    a.grp_zero = (int)st->n_grp;
     a.pair_groups = st->pair_groups;
     a.pair_seed = st->cfg.seed;
     return a;

@@ -247,6 +247,8 @@ int pspin_device_info(int dev, size_t* free_bytes, size_t* total_bytes, char* na
 #define PSPIN_KERNEL_AUTO 0
 #define PSPIN_KERNEL_LISTS 1  /* per-site (partner, conj, conj, g) lists       */
 #define PSPIN_KERNEL_PAIRS 2  /* dense FC per-site pair-row layout             */
+#define PSPIN_KERNEL_GROUPS 3 /* dense: per-site chunks of up to 4 partners
+                               * sharing one conjugated pair (any channels)   */
 #define PSPIN_COUPLING_F32 0
 #define PSPIN_COUPLING_BF16X2 1 /* BF16 hi/lo split, FP32 accumulation       */
 int pspin_set_kernel(pspin_state* st, int kernel, int block_size);
@@ -266,7 +268,8 @@ int pspin_set_pair_schedule(pspin_state* st, int groups);
  * picks it. */
 int pspin_set_pair_tiles(pspin_state* st, int nwaves, int rows);
 /* Coupling entries in flight per thread in the list kernel (8 or 16; 16 for
- * the dense variant only).  pspin_autotune picks it. */
+ * the dense variant only).  The group kernel keeps 4 (batch 8) or 6 (batch
+ * 16) chunks of up to 4 entries in flight.  pspin_autotune picks it. */
 int pspin_set_list_batch(pspin_state* st, int batch);
 int pspin_get_pair_keys(const pspin_state* st, uint64_t* keys);
 /* Timing autotuner for the sweep (replaces optimal_block_size_for_sweep,
@@ -275,8 +278,8 @@ int pspin_get_pair_keys(const pspin_state* st, uint64_t* keys);
  * state after every trial, keeps the fastest.  verbose prints one
  * "[auto-tune]" line per candidate.  Outputs may be NULL. */
 int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, int* block, double* ms_per_sweep);
-/* Coupling storage of the list kernel: FP32 (default) or BF16 hi/lo pairs
- * (g ~ hi + lo, ~16 mantissa bits) with FP32 accumulation. */
+/* Coupling storage of the list and group kernels: FP32 (default) or BF16
+ * hi/lo pairs (g ~ hi + lo, ~16 mantissa bits) with FP32 accumulation. */
 int pspin_set_coupling_format(pspin_state* st, int format);
 
 /* ------------------------------------------------------------------------

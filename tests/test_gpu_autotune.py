@@ -19,9 +19,9 @@ def test_autotune_restores_state_and_picks_a_candidate(pspin, oracle, N, fmc):
     print(f"  N={N} fmc={fmc}: kernel {kernel} block {block} {ms:.4f} ms/sweep")
     assert ms > 0
     assert st.info.kernel == kernel
-    assert kernel in (pspin.KERNEL_LISTS, pspin.KERNEL_PAIRS)
+    assert kernel in (pspin.KERNEL_LISTS, pspin.KERNEL_PAIRS, pspin.KERNEL_GROUPS)
     if fmc != 0:
-        assert kernel == pspin.KERNEL_LISTS
+        assert kernel in (pspin.KERNEL_LISTS, pspin.KERNEL_GROUPS)
     assert block % 64 == 0 and 64 <= block <= 1024
     assert np.array_equal(st.spins(), s0)
     E1, a1, p1 = st.results()

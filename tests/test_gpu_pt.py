@@ -90,7 +90,7 @@ def test_pt_run_counts_and_betas(pspin, oracle):
     st.close()
 
 
-@pytest.mark.parametrize("N,NT,nr,kernel", [(16, 4, 2, 0), (64, 32, 8, 2)])
+@pytest.mark.parametrize("N,NT,nr,kernel", [(16, 4, 2, 0), (64, 32, 8, 2), (32, 8, 4, 3)])
 def test_launch_modes_give_identical_chains(pspin, oracle, N, NT, nr, kernel):
     # Whole PT rounds run as one cooperative launch with a grid barrier (auto),
     # as captured graphs, or as direct launches (with the exchange in the

@@ -121,6 +121,13 @@ def replay_check(O, st, n_sweeps, betas):
     ("dense comb N=128 FP32", 0, 1, dict(N=128, fmc_mode=1, gamma=1.0), 0.3, 1, 0),
     ("dense uniform N=32 BF16 hi/lo", 0, 1, dict(N=32, fmc_mode=2, gamma=0.05), 0.4, 3, 1),
     ("sparse cube N=16 BF16 hi/lo", 1, 1, dict(N=16, fmc_mode=2, gamma=0.05), 0.5, 10, 1),
+    ("dense FC N=12 groups", 0, 3, dict(N=12, fmc_mode=0), 1.0, 6, 0),
+    ("dense uniform N=32 groups (config-1 system)", 0, 3, dict(N=32, fmc_mode=2, gamma=0.05), 0.4, 3, 0),
+    ("dense comb N=16 groups", 0, 3, dict(N=16, fmc_mode=1, gamma=1.0), 0.8, 4, 0),
+    ("dense uniform N=64 groups (learned-proposal system)", 0, 3, dict(N=64, fmc_mode=2, gamma=0.05), 0.3, 1, 0),
+    ("dense FC N=33 odd groups, J0", 0, 3, dict(N=33, fmc_mode=0, J0=0.4, alpha0=0.5), 0.7, 2, 0),
+    ("dense comb N=128 groups BF16 hi/lo (config-4 system)", 0, 3, dict(N=128, fmc_mode=1, gamma=1.0), 0.3, 1, 1),
+    ("dense comb N=128 groups FP32", 0, 3, dict(N=128, fmc_mode=1, gamma=1.0), 0.3, 1, 0),
 ])
 def test_step_replay(pspin, oracle, name, variant, kernel, kw, T, nsw, fmt):
     print(f"[step replay] {name}", flush=True)
@@ -145,6 +152,8 @@ def test_step_replay(pspin, oracle, name, variant, kernel, kw, T, nsw, fmt):
     ("dense FC N=64 pairs, 8 pair groups, 1 wave x 8 rows", 2, dict(N=64, fmc_mode=0), 8, (1, 8)),
     ("dense uniform N=32 lists, 4 pair groups", 1, dict(N=32, fmc_mode=2, gamma=0.05), 4, None),
     ("dense uniform N=64 lists, 16 entries in flight", 1, dict(N=64, fmc_mode=2, gamma=0.05), 0, 16),
+    ("dense uniform N=64 groups, 8 pair groups, 6 chunks in flight", 3, dict(N=64, fmc_mode=2, gamma=0.05), 8, 16),
+    ("dense uniform N=32 groups, 4 pair groups", 3, dict(N=32, fmc_mode=2, gamma=0.05), 4, None),
 ])
 def test_step_replay_shared_pair_schedule(pspin, oracle, name, kernel, kw, groups, tiles):
     """Replicas r = g (mod G) draw their pairs from group g's stream; the rest
