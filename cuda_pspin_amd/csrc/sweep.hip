@@ -390,6 +390,48 @@ __device__ __forceinline__ Prop propose(const StepPre& P, double2 ai, double2 aj
     return q;
 }
 
+// The per-step values every wave reads at the head of a step.
+struct StepShared {
+    StepPre pre[kMaxN / 2];
+    Prop prop[3][2];                 // [step mod 3][outcome of the step before]
+    __align__(16) float part[2][16]; // [step mod 2][wave] partial Delta E; unused slots stay 0
+};
+
+// Step-head gather: five 4-byte LDS reads per lane, each from a wave-uniform
+// base plus a lane offset, all issued together (one round trip):
+//   hyp   lanes 0-15: mi, mj, di, dj of step sn's proposal, hypothesis 0 then 1
+//   part  lanes 0-15: the 16 partial Delta E of step sp
+//   res   lanes 0-7:  oi, oj, fi, fj of step sp's proposal (outcome hyp)
+//   pres  lanes 0-15: step sp's StepPre words (i0, j0, ..., thr in 14-15)
+//   pren  lanes 0-1:  i0, j0 of step sn
+struct Head {
+    uint32_t hyp, part, res, pres, pren;
+};
+
+__device__ __forceinline__ Head head_gather(const StepShared& ss, int lane, int sn, int sp, int hyp) {
+    const int l16 = lane & 15, l8 = lane & 7;
+    Head h;
+    h.hyp = ((const uint32_t*)&ss.prop[sn % 3][(lane >> 3) & 1].mi)[l8];
+    h.part = ((const uint32_t*)ss.part[sp & 1])[l16];
+    h.res = ((const uint32_t*)&ss.prop[sp % 3][hyp].oi)[l8];
+    h.pres = ((const uint32_t*)&ss.pre[sp])[l16];
+    h.pren = ((const uint32_t*)&ss.pre[sn])[lane & 1];
+    return h;
+}
+
+__device__ __forceinline__ float lanef(uint32_t v, int l) { return __uint_as_float(__builtin_amdgcn_readlane(v, l)); }
+
+// Sum of the 16 partials in lanes 0-15 (one DPP row) in a fixed order:
+// pairs, quads, then rows of 8 and 16 by rotation; read from lane 0.
+__device__ __forceinline__ float head_sum16(uint32_t hv) {
+    float v = __uint_as_float(hv);
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xb1, 0xf, 0xf, false));  // quad_perm 1,0,3,2
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4e, 0xf, 0xf, false));  // quad_perm 2,3,0,1
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false)); // row_ror 4
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false)); // row_ror 8
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+}
+
 // Step pipeline, one workgroup barrier per step.  During step t's field
 // phase, lanes 0 and 1 of wave 0 compute step t+1's proposal under the two
 // outcomes of step t (rejected, accepted): the pairs of consecutive steps may
@@ -405,9 +447,10 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     constexpr bool PAIRS = LAYOUT == kPairs, GROUPS = LAYOUT == kGroups;
     __shared__ double2 s64[kMaxN];
     __shared__ __align__(16) float2 view[16][kMaxN];
-    __shared__ StepPre pre[kMaxN / 2];
-    __shared__ float part[2][16];
-    __shared__ Prop prop[3][2];  // [step mod 3][outcome of the step before]
+    __shared__ StepShared ss;
+    StepPre* pre = ss.pre;
+    Prop(*prop)[2] = ss.prop;
+    float(*part)[16] = ss.part;
     __shared__ int last_wg;
     __shared__ unsigned int gen_seen;
     __shared__ int goff[kMaxN + 1];  // GROUPS: chunk offsets per site
@@ -423,6 +466,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     float2* s32 = view[w];
 
     for (int i = tid; i < N; i += bdim) s64[i] = make_double2(gs[2 * i], gs[2 * i + 1]);
+    if (tid < 32) part[tid >> 4][tid & 15] = 0.f;
     // pad entries are read by zero-coupling rows of the pair tiles
     for (int i = lane; i < ((N + 3) & ~3); i += 64)
         s32[i] = i < N ? make_float2((float)gs[2 * i], (float)gs[2 * i + 1]) : make_float2(0.f, 0.f);
@@ -437,6 +481,9 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         for (int i = tid; i < N * N; i += bdim) g2s[i] = a.g2[i];
     if (GROUPS)
         for (int i = tid; i <= N; i += bdim) goff[i] = a.grp_off[i];
+    // GROUPS: chunks are dealt from the last thread down, so a partial last
+    // batch leaves wave 0 (which also forms the next proposal) the least work
+    const int gt = bdim - 1 - tid;
     GrpBuf<PB> gB;
 
     // PAIRS: this wave's batch descriptors, one per lane (an even count)
@@ -461,7 +508,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             const GrpRange r0 = grp_range(goff, __builtin_amdgcn_readfirstlane(pre[0].i0),
                                           __builtin_amdgcn_readfirstlane(pre[0].j0));
 #pragma unroll
-            for (int u = 0; u < PB; u++) grp_load<PB>(a, gB, u, r0, tid + u * bdim);
+            for (int u = 0; u < PB; u++) grp_load<PB>(a, gB, u, r0, gt + u * bdim);
         }
         StepRec* logp = nullptr;
         const int slog = a.log_base + rd * a.n_sweeps + sw;
@@ -474,19 +521,32 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
 
         int hyp = 0;  // outcome of the step before the current one (selects its proposal)
         for (int step = 0; step <= nsteps; step++) {
+            // Step head.  Every LDS value it needs has an address known before
+            // the decision (both hypotheses of this step's proposal, both
+            // outcomes of the step resolved, the partial sums, sites and
+            // threshold), so it is gathered in one round trip after the
+            // barrier and taken by readlane (SGPRs, not VGPRs).
+            const bool more = step < nsteps;
+            const int sn = more ? step : 0;
+            const int sp = step > 0 ? step - 1 : 0;  // the step resolved here (none at step 0)
+            const Head hv = head_gather(ss, lane, sn, sp, hyp);
             if (step > 0) {
-                // Resolve step-1 (src/mc.cu:284-304).
-                const int sp = step - 1;
-                const StepPre& Pp = pre[sp];
+                // Resolve step-1 (src/mc.cu:284-304).  The 16 partials are summed
+                // in a fixed order and read from one lane, so every wave takes the
+                // same decision.
+                const float dE = head_sum16(hv.part);
+                const double thr = __hiloint2double((int)__builtin_amdgcn_readlane(hv.pres, 15),
+                                                    (int)__builtin_amdgcn_readlane(hv.pres, 14));
+                const int ri = __builtin_amdgcn_readlane(hv.pres, 0);
+                const int rj = __builtin_amdgcn_readlane(hv.pres, 1);
+                const bool accept = (beta == 0.0) | ((double)dE < thr);
+                const int fo = accept ? 4 : 0;  // fi, fj if accepted, else oi, oj
+                s32[ri] = make_float2(lanef(hv.res, fo), lanef(hv.res, fo + 1));
+                s32[rj] = make_float2(lanef(hv.res, fo + 2), lanef(hv.res, fo + 3));
                 const Prop& Q = prop[sp % 3][hyp];
-                float dE = 0.f;
-                for (int k = 0; k < nw; k++) dE += part[sp & 1][k];
-                const bool accept = (beta == 0.0) || ((double)dE < Pp.thr);
-                s32[Pp.i0] = accept ? Q.fi : Q.oi;
-                s32[Pp.j0] = accept ? Q.fj : Q.oj;
                 if (w == 0 && accept) {
-                    s64[Pp.i0] = Q.ni;
-                    s64[Pp.j0] = Q.nj;
+                    s64[ri] = Q.ni;
+                    s64[rj] = Q.nj;
                 }
                 if (tid == 0) {
                     if (accept) {
@@ -496,7 +556,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                     nprop++;
                     if (logp) {
                         StepRec r;
-                        r.i0 = Pp.i0; r.j0 = Pp.j0; r.acc = accept; r.dE = dE;
+                        r.i0 = ri; r.j0 = rj; r.acc = accept; r.dE = dE;
                         r.ai[0] = Q.ni.x; r.ai[1] = Q.ni.y;
                         r.aj[0] = Q.nj.x; r.aj[1] = Q.nj.y;
                         logp[sp] = r;
@@ -504,15 +564,16 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 }
                 hyp = accept;
             }
-            if (step == nsteps) break;
+            if (!more) break;
 
-            const StepPre& P = pre[step];
-            const Prop& Q = prop[step % 3][hyp];
             // wave-uniform: the address arithmetic of the tiles stays scalar
-            const int i0 = __builtin_amdgcn_readfirstlane(P.i0), j0 = __builtin_amdgcn_readfirstlane(P.j0);
-            const float2 di = Q.di, dj = Q.dj;
-            s32[i0] = Q.mi;
-            s32[j0] = Q.mj;
+            const int i0 = __builtin_amdgcn_readlane(hv.pren, 0), j0 = __builtin_amdgcn_readlane(hv.pren, 1);
+            const int mo = 8 * hyp;  // mi, mj, di, dj of this step's proposal
+            const float2 di = make_float2(lanef(hv.hyp, mo + 4), lanef(hv.hyp, mo + 5));
+            const float2 dj = make_float2(lanef(hv.hyp, mo + 6), lanef(hv.hyp, mo + 7));
+            s32[i0] = make_float2(lanef(hv.hyp, mo), lanef(hv.hyp, mo + 1));
+            s32[j0] = make_float2(lanef(hv.hyp, mo + 2), lanef(hv.hyp, mo + 3));
+            const Prop& Qs = prop[step % 3][hyp];
             float local = 0.f;
             if (a.h2_active && w == nw - 1) {
                 float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
@@ -561,10 +622,10 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 for (int b = 0; b < nbt; b++) {
                     const bool same = b + 1 < nbt;
                     const GrpRange& rl = same ? r : rn;
-                    const int e1 = same ? tid + (b + 1) * per : tid;
+                    const int e1 = same ? gt + (b + 1) * per : gt;
 #pragma unroll
                     for (int u = 0; u < PB; u++) {
-                        local += grp_chunk<BF16, PB>(gB, u, s32, tid + b * per + u * bdim >= r.ci, di, dj);
+                        local += grp_chunk<BF16, PB>(gB, u, s32, gt + b * per + u * bdim >= r.ci, di, dj);
                         grp_load<PB>(a, gB, u, rl, e1 + u * bdim);  // (after a sweep's last step: unused)
                     }
                 }
@@ -577,8 +638,8 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 const StepPre& Pn = pre[step + 1];
                 double2 ai = s64[Pn.i0], aj = s64[Pn.j0];
                 if (lane == 1) {
-                    if (Pn.i0 == i0) ai = Q.ni; else if (Pn.i0 == j0) ai = Q.nj;
-                    if (Pn.j0 == i0) aj = Q.ni; else if (Pn.j0 == j0) aj = Q.nj;
+                    if (Pn.i0 == i0) ai = Qs.ni; else if (Pn.i0 == j0) ai = Qs.nj;
+                    if (Pn.j0 == i0) aj = Qs.ni; else if (Pn.j0 == j0) aj = Qs.nj;
                 }
                 prop[(step + 1) % 3][lane] = propose<CUBE>(Pn, ai, aj);
             }
