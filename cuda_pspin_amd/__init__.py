@@ -140,7 +140,6 @@ def load_library() -> C.CDLL:
         "pspin_set_kernel": (C.c_int, [P, C.c_int, C.c_int]),
         "pspin_set_pair_schedule": (C.c_int, [P, C.c_int]),
         "pspin_set_pair_tiles": (C.c_int, [P, C.c_int, C.c_int]),
-        "pspin_set_pair_replicas": (C.c_int, [P, C.c_int]),
         "pspin_set_list_batch": (C.c_int, [P, C.c_int]),
         "pspin_get_pair_keys": (C.c_int, [P, u64p]),
         "pspin_autotune": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -390,10 +389,6 @@ class PSpin:
 
     def set_pair_tiles(self, nwaves: int, rows: int) -> None:
         _check(self._L.pspin_set_pair_tiles(self._h, int(nwaves), int(rows)))
-
-    def set_pair_replicas(self, reps: int) -> None:
-        """Replicas per workgroup of the pair-row kernel (1, 2, 4; needs a pair schedule)."""
-        _check(self._L.pspin_set_pair_replicas(self._h, int(reps)))
 
     def set_list_batch(self, batch: int) -> None:
         _check(self._L.pspin_set_list_batch(self._h, int(batch)))

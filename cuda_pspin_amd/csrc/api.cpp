@@ -1178,14 +1178,6 @@ int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweep, dou
     return PSPIN_OK;
 }
 
-// K replicas share a workgroup of the pair-row kernel only when they share a
-// pair stream: groups > 0 and every workgroup's K replicas in one group.
-constexpr int kMultiMaxWaves = 8;  // the K-replica pair-row kernel's workgroups have at most 512 threads
-
-static bool pair_reps_ok(const pspin_state* st, int groups, int reps) {
-    return reps == 1 || (groups > 0 && st->nrep % (groups * reps) == 0);
-}
-
 static const char* kernel_name(int k) {
     return k == PSPIN_KERNEL_PAIRS ? "pair-row" : k == PSPIN_KERNEL_GROUPS ? "group" : "list";
 }
@@ -1301,30 +1293,26 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
         ps::set_error(std::string("HIP error in pspin_autotune: ") + hipGetErrorString(e));
         return PSPIN_ERR_HIP;
     }
-    struct Cand { int kernel, param, rows, reps; };
+    struct Cand { int kernel, param, rows; };
     std::vector<Cand> cands;
     const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
     if (st->d_ptab && !bf16 && st->variant == PSPIN_DENSE)
-        for (int reps : {1, 2, 4}) {
-            if (reps > 1 && !pair_reps_ok(st, st->pair_groups, reps)) continue;
-            for (int rows : {4, 8})
-                for (int nw : {4, 8, 12, 16})
-                    if (reps == 1 || nw <= kMultiMaxWaves) cands.push_back({PSPIN_KERNEL_PAIRS, nw, rows, reps});
-        }
+        for (int rows : {4, 8})
+            for (int nw : {4, 8, 12, 16}) cands.push_back({PSPIN_KERNEL_PAIRS, nw, rows});
     for (int lb : {8, 16}) {
         if (lb == 16 && st->variant != PSPIN_DENSE) continue;
-        for (int bs : {64, 128, 256, 512, 1024}) cands.push_back({PSPIN_KERNEL_LISTS, bs, lb, 1});
+        for (int bs : {64, 128, 256, 512, 1024}) cands.push_back({PSPIN_KERNEL_LISTS, bs, lb});
         if (st->d_grp_off)
-            for (int bs : {128, 256, 512, 1024}) cands.push_back({PSPIN_KERNEL_GROUPS, bs, lb, 1});
+            for (int bs : {128, 256, 512, 1024}) cands.push_back({PSPIN_KERNEL_GROUPS, bs, lb});
     }
     const int old_kernel = st->kernel, old_block = st->block_size, old_nw = st->pair_nwaves,
-              old_rows = st->pair_rows, old_lb = st->list_batch, old_reps = st->pair_reps;
+              old_rows = st->pair_rows, old_lb = st->list_batch;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     double best = 1e300;
     Cand pick = {old_kernel, old_kernel == PSPIN_KERNEL_PAIRS ? old_nw : old_block,
-                 old_kernel == PSPIN_KERNEL_PAIRS ? old_rows : old_lb, old_reps};
+                 old_kernel == PSPIN_KERNEL_PAIRS ? old_rows : old_lb};
     int rc = PSPIN_OK;
     for (const Cand& c : cands) {
         if (c.kernel == PSPIN_KERNEL_PAIRS) {
@@ -1334,7 +1322,6 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
                 continue;
             }
             if (rc) break;
-            st->pair_reps = c.reps;
         } else {
             st->block_size = c.param;
             st->list_batch = c.rows;
@@ -1355,9 +1342,9 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
         }
         const double per = ms / trial_sweeps;
         if (verbose)
-            std::printf("[auto-tune] sweep N=%d nrep=%d %s %s=%d depth=%d replicas/wg=%d: %.4f ms/sweep\n", st->N,
-                        st->nrep, kernel_name(c.kernel), c.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block", c.param,
-                        c.rows, c.reps, per);
+            std::printf("[auto-tune] sweep N=%d nrep=%d %s %s=%d depth=%d: %.4f ms/sweep\n", st->N, st->nrep,
+                        kernel_name(c.kernel), c.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block",
+                        c.param, c.rows, per);
         if (per < best) {
             best = per;
             pick = c;
@@ -1369,11 +1356,9 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
         st->kernel = pick.kernel;
         if (pick.kernel == PSPIN_KERNEL_PAIRS) {
             rc = build_pair_tasks(st, pick.param, pick.rows);
-            st->pair_reps = pick.reps;
         } else {
             st->block_size = pick.param;
             st->list_batch = pick.rows;
-            st->pair_reps = old_reps;
             if (st->d_ptab && (st->pair_nwaves != old_nw || st->pair_rows != old_rows))
                 rc = build_pair_tasks(st, old_nw, old_rows);
         }
@@ -1381,7 +1366,6 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
         st->kernel = old_kernel;
         st->block_size = old_block;
         st->list_batch = old_lb;
-        st->pair_reps = old_reps;
         if (st->d_ptab) (void)build_pair_tasks(st, old_nw, old_rows);
     }
     (void)hipStreamSynchronize(st->stream);
@@ -1389,9 +1373,9 @@ int pspin_autotune(pspin_state* st, int trial_sweeps, int verbose, int* kernel, 
     drop_graph(st);
     if (rc) return rc;
     if (verbose)
-        std::printf("[auto-tune] sweep N=%d -> %s %s=%d depth=%d replicas/wg=%d (%.4f ms/sweep)\n", st->N,
-                    kernel_name(pick.kernel), pick.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block", pick.param,
-                    pick.rows, pick.reps, best);
+        std::printf("[auto-tune] sweep N=%d -> %s %s=%d depth=%d (%.4f ms/sweep)\n", st->N,
+                    kernel_name(pick.kernel),
+                    pick.kernel == PSPIN_KERNEL_PAIRS ? "waves" : "block", pick.param, pick.rows, best);
     if (kernel) *kernel = pick.kernel;
     if (block) *block = pick.kernel == PSPIN_KERNEL_PAIRS ? 64 * pick.param : pick.param;
     if (ms_per_sweep) *ms_per_sweep = best;
@@ -1402,8 +1386,6 @@ int pspin_set_pair_tiles(pspin_state* st, int nwaves, int rows) {
     PS_REQUIRE(st && st->d_ptab, PSPIN_ERR_STATE, "no pair-row layout (dense fully connected, 8 <= N <= 128)");
     PS_REQUIRE(nwaves >= 1 && nwaves <= 16 && (rows == 4 || rows == 8), PSPIN_ERR_ARG,
                "nwaves in [1, 16], rows 4 or 8");
-    PS_REQUIRE(st->pair_reps == 1 || nwaves <= kMultiMaxWaves, PSPIN_ERR_ARG,
-               "at most 8 waves with several replicas per workgroup");
     drop_graph(st);
     return build_pair_tasks(st, nwaves, rows);
 }
@@ -1420,20 +1402,6 @@ int pspin_set_pair_schedule(pspin_state* st, int groups) {
     PS_REQUIRE(st && groups >= 0 && groups <= st->nrep, PSPIN_ERR_ARG, "groups must be in [0, nrep]");
     drop_graph(st);
     st->pair_groups = groups;
-    if (!pair_reps_ok(st, groups, st->pair_reps)) st->pair_reps = 1;  // the schedule no longer supports them
-    return PSPIN_OK;
-}
-
-int pspin_set_pair_replicas(pspin_state* st, int reps) {
-    PS_REQUIRE(st && (reps == 1 || reps == 2 || reps == 4), PSPIN_ERR_ARG, "replicas per workgroup must be 1, 2 or 4");
-    PS_REQUIRE(reps == 1 || st->d_ptab, PSPIN_ERR_STATE, "no pair-row layout (dense fully connected, 8 <= N <= 128)");
-    PS_REQUIRE(pair_reps_ok(st, st->pair_groups, reps), PSPIN_ERR_STATE,
-               "several replicas per workgroup need a shared pair schedule (pspin_set_pair_schedule) with nrep a "
-               "multiple of groups x replicas");
-    PS_REQUIRE(reps == 1 || st->pair_nwaves <= kMultiMaxWaves, PSPIN_ERR_STATE,
-               "several replicas per workgroup run at most 8 waves (pspin_set_pair_tiles)");
-    drop_graph(st);
-    st->pair_reps = reps;
     return PSPIN_OK;
 }
 

@@ -214,7 +214,6 @@ struct pspin_state {
     int* d_wave_tasks = nullptr;
     int pair_nwaves = 0;
     int pair_rows = 16; // rows in flight per batch in the pair-row kernel (8 or 16)
-    int pair_reps = 1;  // replicas per workgroup in the pair-row kernel (1, 2 or 4; >1 needs a pair schedule)
     int list_batch = 8; // entries in flight per thread in the list kernel (8 or 16; dense only)
 
     // replicas
@@ -314,8 +313,6 @@ hipError_t launch_sweep(pspin_state* st, int n_sweeps, bool exchange = false);
 // workgroup can be resident at once (and the exchange can be fused).
 bool pt_rounds_ok(const pspin_state* st);
 hipError_t launch_pt_rounds(pspin_state* st, int pt_freq, int rounds);
-// Workgroups of a sweep launch (replicas / replicas per workgroup).
-int sweep_grid(const pspin_state* st);
 
 hipError_t launch_exchange(pspin_state* st);
 

@@ -711,361 +711,6 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// Fully connected systems, K replicas per workgroup.  With a shared pair
-// schedule (pair_groups = G > 0) the replicas r = g (mod G) visit the same
-// sites at every step, so one workgroup can serve K of them and load each
-// coupling row once for all K: the tiles stream from L2 to the CU at a
-// per-CU rate, and the K-fold reuse cuts the bytes per pair-update K-fold.
-// Workgroup q serves replicas (q mod G) + G (K floor(q / G) + k), k < K: all
-// of group q mod G, and with G = 8 on the XCD that group's tiles stay in.
-// The step pipeline is the one of k_sweep, per replica: every replica has
-// its own amplitudes, draws, threshold, proposals, decision and energy; the
-// pair, the tiles and the barrier are shared.  Wave 0 lanes 2k, 2k + 1 form
-// replica k's next proposal; thread k keeps replica k's energy and counters.
-// ---------------------------------------------------------------------------
-struct MultiShared {
-    StepPre pre[4][kMaxPairsN / 2];  // [replica][step]
-    Prop prop[4][3][2];              // [replica][step mod 3][outcome of the step before]
-    __align__(16) float part[2][4][16];  // [step mod 2][replica][wave]; unused slots stay 0
-};
-
-__device__ __forceinline__ int multi_rep(int q, int k, int G, int K) { return q % G + G * (K * (q / G) + k); }
-
-// Sum of the 16 partials of each DPP row (replica k in row k) in a fixed
-// order; replica k's sum is then read from lane 16 k.
-__device__ __forceinline__ float row_sums16(float v) {
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xb1, 0xf, 0xf, false));  // quad_perm 1,0,3,2
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4e, 0xf, 0xf, false));  // quad_perm 2,3,0,1
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false)); // row_ror 4
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false)); // row_ror 8
-    return v;
-}
-
-// v[k] for a k known only at run time (no indexed register array).
-template <int K, class T>
-__device__ __forceinline__ T pick(const T (&v)[K], int k) {
-    T r = v[0];
-#pragma unroll
-    for (int j = 1; j < K; j++)
-        if (k == j) r = v[j];
-    return r;
-}
-
-// pair_step for K replicas: one load of each coupling row, K amplitude rows.
-template <int PB, int K>
-__device__ __forceinline__ void pair_step_k(const SweepArgs& a, PairBuf<PB>& B, int4 bd, int4 bn, int ni, int nj,
-                                            const float2* s32, const float2 (&di)[K], const float2 (&dj)[K],
-                                            float2 (&acc)[K][4], uint4& inf, int lane, float (&local)[K]) {
-    const int nr = (bd.x >> 8) & 15, nn = (bn.x >> 8) & 15;
-    const int nsite = (bn.x & 1) ? nj : ni;
-    if (bd.x & 2) {
-        inf = B.inf;
-#pragma unroll
-        for (int k = 0; k < K; k++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[k][j] = make_float2(0.f, 0.f);
-    }
-    if (bn.x & 2) B.inf = *(const uint4*)(a.pinfo + (size_t)nsite * a.npad + bn.y + 4 * lane);
-#pragma unroll
-    for (int u = 0; u < PB; u += 4) {
-        if (u < nr && lane < quad_lanes(bd.x, u / 4)) {
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const float4* tp = (const float4*)(s32 + k * kMaxPairsN + bd.z);
-                const float4 t01 = tp[u / 2], t23 = tp[u / 2 + 1];
-                const float ta[8] = {t01.x, t01.y, t01.z, t01.w, t23.x, t23.y, t23.z, t23.w};
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const float4 q = B.g[u + v];
-                    const float tx = ta[2 * v], ty = ta[2 * v + 1];
-                    acc[k][0].x += q.x * tx; acc[k][0].y += q.x * ty;
-                    acc[k][1].x += q.y * tx; acc[k][1].y += q.y * ty;
-                    acc[k][2].x += q.z * tx; acc[k][2].y += q.z * ty;
-                    acc[k][3].x += q.w * tx; acc[k][3].y += q.w * ty;
-                }
-            }
-        }
-        if (u < nn) pair_load_quad<PB>(a, B, bn, nsite, lane, u);
-    }
-    if (!(bd.x & 4)) return;
-    const uint32_t e[4] = {inf.x, inf.y, inf.z, inf.w};
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        float2 Hc = make_float2(0.f, 0.f);
-        const char* sb = (const char*)(s32 + k * kMaxPairsN);
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const float2 tb = *(const float2*)(sb + (e[j] & 0xffffu));
-            const float2 tc = *(const float2*)(sb + ((e[j] >> 16) & 0x7fffu));
-            const float zx = tc.x * acc[k][j].x + tc.y * acc[k][j].y;
-            const float zy =
-                __uint_as_float(__float_as_uint(tc.x * acc[k][j].y - tc.y * acc[k][j].x) ^ (e[j] & 0x80000000u));
-            Hc.x += tb.x * zx - tb.y * zy;
-            Hc.y += tb.x * zy + tb.y * zx;
-        }
-        const float2 d = (bd.x & 1) ? dj[k] : di[k];
-        local[k] += -(d.x * Hc.x + d.y * Hc.y);  // the expression of pair_step, so K = 1 sums agree
-    }
-}
-
-template <int PB, int K>
-__global__ __launch_bounds__(512) void k_sweep_pairs_k(SweepArgs a) {
-    constexpr int NP = kMaxPairsN;
-    __shared__ double2 s64[K][NP];
-    __shared__ __align__(16) float2 view[16][K][NP];
-    __shared__ MultiShared ms;
-    __shared__ int last_wg;
-    __shared__ unsigned int gen_seen;
-    extern __shared__ float g2s[];  // [N][N] two-body couplings when g2_lds
-
-    const int q = blockIdx.x, G = a.pair_groups;
-    const int tid = threadIdx.x, bdim = blockDim.x;
-    const int nw = bdim >> 6, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    const int N = a.N, nsteps = N / 2;
-    float2* s32 = &view[w][0][0];  // replica k's copy at s32 + k NP
-    int rep[K];
-    double beta[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        rep[k] = multi_rep(q, k, G, K);
-        beta[k] = a.betas[rep[k]];
-        const double* gs = a.spins + (size_t)2 * N * rep[k];
-        for (int i = tid; i < N; i += bdim) s64[k][i] = make_double2(gs[2 * i], gs[2 * i + 1]);
-        for (int i = lane; i < ((N + 3) & ~3); i += 64)
-            s32[k * NP + i] = i < N ? make_float2((float)gs[2 * i], (float)gs[2 * i + 1]) : make_float2(0.f, 0.f);
-    }
-    for (int i = tid; i < 2 * 4 * 16; i += bdim) (&ms.part[0][0][0])[i] = 0.f;
-    // thread k < K keeps replica k's energy, counters and log
-    const int kt = tid < K ? tid : 0;
-    const int rep_t = multi_rep(q, kt, G, K);
-    const uint64_t sweep0 = a.ctr[rep[0]];  // every replica's counter advances with every launch
-    const uint64_t gkey = pair_group_key(a.pair_seed, q % G);
-    double E = 0.0;
-    long long nacc = 0, nprop = 0;
-    if (tid < K) E = a.E[rep_t];
-    if (a.h2_active && a.g2_lds)
-        for (int i = tid; i < N * N; i += bdim) g2s[i] = a.g2[i];
-
-    int4 desc = make_int4(0, 0, 0, 0);
-    const int t0 = a.wave_tasks[w];
-    const int nb = a.wave_tasks[w + 1] - t0;
-    if (lane < nb) desc = a.tasks[t0 + lane];
-    PairBuf<PB> bA;
-
-    for (int rd = 0; rd < a.rounds; rd++) {
-    for (int sw = 0; sw < a.n_sweeps; sw++) {
-        const uint64_t sweep = sweep0 + (uint64_t)rd * a.n_sweeps + (uint64_t)sw;
-        for (int x = tid; x < K * nsteps; x += bdim) {
-            const int k = x / nsteps, st = x - k * nsteps;
-            ms.pre[k][st] = make_step<false>(a.seeds[multi_rep(q, k, G, K)], gkey, sweep, st, N, pick<K>(beta, k));
-        }
-        __syncthreads();
-        if (nb > 0)
-            pair_issue<PB>(a, bA, desc_at(desc, 0), __builtin_amdgcn_readfirstlane(ms.pre[0][0].i0),
-                           __builtin_amdgcn_readfirstlane(ms.pre[0][0].j0), lane);
-        StepRec* logp = nullptr;
-        const int slog = a.log_base + rd * a.n_sweeps + sw;
-        if (a.log && tid < K && slog < a.log_cap) logp = a.log + ((size_t)rep_t * a.log_cap + slog) * nsteps;
-        if (w == 0 && lane < 2 * K) {
-            const int k = lane >> 1;
-            const StepPre& P = ms.pre[k][0];
-            ms.prop[k][0][lane & 1] = propose<false>(P, s64[k][P.i0], s64[k][P.j0]);
-        }
-        __syncthreads();
-
-        int hyp[K];
-#pragma unroll
-        for (int k = 0; k < K; k++) hyp[k] = 0;
-        int r_sp = 0, r_st = 0, r_nx = 1;
-        for (int step = 0; step <= nsteps; step++) {
-            // Step head, as in k_sweep, for K replicas: lane 16 k + l gathers
-            // replica k's partial of wave l and word l of its resolved StepPre.
-            const bool more = step < nsteps;
-            const int sn = more ? step : 0;
-            const int sp = step > 0 ? step - 1 : 0;
-            const int kl = (lane >> 4) & (K - 1), l16 = lane & 15;
-            const float pv = ms.part[sp & 1][kl][l16];
-            const uint32_t pres = ((const uint32_t*)&ms.pre[kl][sp])[l16];
-            const uint32_t pren = ((const uint32_t*)&ms.pre[0][sn])[lane & 1];
-            float4 o01[K], f01[K];
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                o01[k] = *(const float4*)&ms.prop[k][r_sp][hyp[k]].oi;
-                f01[k] = *(const float4*)&ms.prop[k][r_sp][hyp[k]].fi;
-            }
-            if (step > 0) {
-                const int ri = __builtin_amdgcn_readlane(pres, 0), rj = __builtin_amdgcn_readlane(pres, 1);
-                const float psum = row_sums16(pv);
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const float dE = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(psum), 16 * k));
-                    const double thr = __hiloint2double((int)__builtin_amdgcn_readlane(pres, 16 * k + 15),
-                                                        (int)__builtin_amdgcn_readlane(pres, 16 * k + 14));
-                    const bool accept = (beta[k] == 0.0) | ((double)dE < thr);
-                    s32[k * NP + ri] = accept ? make_float2(f01[k].x, f01[k].y) : make_float2(o01[k].x, o01[k].y);
-                    s32[k * NP + rj] = accept ? make_float2(f01[k].z, f01[k].w) : make_float2(o01[k].z, o01[k].w);
-                    const Prop& Q = ms.prop[k][r_sp][hyp[k]];
-                    if (w == 0 && accept) {
-                        const double2 ni = Q.ni, nj = Q.nj;
-                        s64[k][ri] = ni;
-                        s64[k][rj] = nj;
-                    }
-                    if (tid == k) {
-                        if (accept) {
-                            E += (double)dE;
-                            nacc++;
-                        }
-                        nprop++;
-                        if (logp) {
-                            StepRec r;
-                            r.i0 = ri; r.j0 = rj; r.acc = accept; r.dE = dE;
-                            r.ai[0] = Q.ni.x; r.ai[1] = Q.ni.y;
-                            r.aj[0] = Q.nj.x; r.aj[1] = Q.nj.y;
-                            logp[sp] = r;
-                        }
-                    }
-                    hyp[k] = __builtin_amdgcn_readfirstlane((int)accept);
-                }
-            }
-            if (!more) break;
-
-            const int i0 = __builtin_amdgcn_readlane(pren, 0), j0 = __builtin_amdgcn_readlane(pren, 1);
-            float2 di[K], dj[K];
-            float local[K];
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const float4 m01 = *(const float4*)&ms.prop[k][r_st][hyp[k]].mi;
-                const float4 d01 = *(const float4*)&ms.prop[k][r_st][hyp[k]].di;
-                di[k] = make_float2(d01.x, d01.y);
-                dj[k] = make_float2(d01.z, d01.w);
-                s32[k * NP + i0] = make_float2(m01.x, m01.y);
-                s32[k * NP + j0] = make_float2(m01.z, m01.w);
-                local[k] = 0.f;
-            }
-            if (a.h2_active && w == nw - 1) {
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
-                    for (int x = lane; x < N; x += 64) {
-                        const float2 t = s32[k * NP + x];
-                        float gi, gj;
-                        if (a.g2_lds) {
-                            gi = x != i0 ? g2s[i0 * N + x] : 0.f;
-                            gj = x != j0 ? g2s[j0 * N + x] : 0.f;
-                        } else {
-                            gi = x != i0 ? a.g2[(size_t)i0 * N + x] : 0.f;
-                            gj = x != j0 ? a.g2[(size_t)j0 * N + x] : 0.f;
-                        }
-                        Fi.x += gi * t.x; Fi.y -= gi * t.y;
-                        Fj.x += gj * t.x; Fj.y -= gj * t.y;
-                    }
-                    local[k] = -(di[k].x * Fi.x - di[k].y * Fi.y) - (dj[k].x * Fj.x - dj[k].y * Fj.y);
-                }
-            }
-            {
-                float2 acc[K][4];
-                uint4 inf = make_uint4(0u, 0u, 0u, 0u);
-                const int ni = __builtin_amdgcn_readfirstlane(step + 1 < nsteps ? ms.pre[0][step + 1].i0 : 0);
-                const int nj = __builtin_amdgcn_readfirstlane(step + 1 < nsteps ? ms.pre[0][step + 1].j0 : 0);
-                int4 bd = nb > 0 ? desc_at(desc, 0) : make_int4(0, 0, 0, 0);
-                for (int b = 0; b < nb; b++) {
-                    int4 bn = make_int4(0, 0, 0, 0);
-                    if (b + 1 < nb) bn = desc_at(desc, b + 1);
-                    else if (step + 1 < nsteps) bn = desc_at(desc, 0);
-                    const bool same = b + 1 < nb;
-                    pair_step_k<PB, K>(a, bA, bd, bn, same ? i0 : ni, same ? j0 : nj, s32, di, dj, acc, inf, lane,
-                                       local);
-                    bd = bn;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const float v = wave_sum(local[k]);
-                if (lane == 0) ms.part[step & 1][k][w] = v;
-            }
-            if (w == 0 && lane < 2 * K && step + 1 < nsteps) {
-                // Replica k's next proposal if this step is rejected (lane 2k) or
-                // accepted (lane 2k + 1).
-                const int k = lane >> 1;
-                const StepPre& Pn = ms.pre[k][step + 1];
-                const Prop& Qs = ms.prop[k][r_st][pick<K>(hyp, k)];
-                double2 ai = s64[k][Pn.i0], aj = s64[k][Pn.j0];
-                if (lane & 1) {
-                    if (Pn.i0 == i0) ai = Qs.ni; else if (Pn.i0 == j0) ai = Qs.nj;
-                    if (Pn.j0 == i0) aj = Qs.ni; else if (Pn.j0 == j0) aj = Qs.nj;
-                }
-                ms.prop[k][r_nx][lane & 1] = propose<false>(Pn, ai, aj);
-            }
-            r_sp = r_st;
-            r_st = r_nx;
-            r_nx = r_nx == 2 ? 0 : r_nx + 1;
-            __syncthreads();
-        }
-        __syncthreads();  // pre[] and prop[] are rewritten by the next sweep
-    }
-    if (a.exchange) {
-        // As in k_sweep: the last workgroup out runs the exchange.
-        const bool more = rd + 1 < a.rounds;
-        if (tid < K) dev_store(a.E + rep_t, E);
-        if (tid == 0) gen_seen = dev_load(a.done + 1);  // before arriving, so the release cannot be missed
-        __syncthreads();  // every replica's energy store has landed
-        if (tid == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            last_wg = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
-        }
-        __syncthreads();
-        if (last_wg) {
-            if (tid == 0) dev_store(a.done, 0u);
-            exchange_stage(a.ex, &view[0][0][0], tid, bdim);
-            __syncthreads();
-            if (w == 0) exchange_wave(a.ex, &view[0][0][0], lane);
-            if (more) {
-                if (w == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                if (tid == 0) __hip_atomic_fetch_add(a.done + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __syncthreads();
-                // the exchange staged its slots over the amplitude copies
-#pragma unroll
-                for (int k = 0; k < K; k++)
-                    for (int i = lane; i < ((N + 3) & ~3); i += 64)
-                        s32[k * NP + i] =
-                            i < N ? make_float2((float)s64[k][i].x, (float)s64[k][i].y) : make_float2(0.f, 0.f);
-            }
-        } else if (more) {
-            if (tid == 0) {
-                for (unsigned n = 0; dev_load(a.done + 1) == gen_seen; n++) {
-                    if (n == kBarrierSpinLimit) {
-                        __hip_atomic_fetch_or(a.done + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
-                }
-            }
-        }
-        if (more) {
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < K; k++) beta[k] = dev_load(a.betas + rep[k]);
-        }
-    }
-    }
-
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double* gs = a.spins + (size_t)2 * N * rep[k];
-        for (int i = tid; i < N; i += bdim) {
-            gs[2 * i] = s64[k][i].x;
-            gs[2 * i + 1] = s64[k][i].y;
-        }
-    }
-    if (tid < K) {
-        a.E[rep_t] = E;
-        a.acc[rep_t] += nacc;
-        a.prop[rep_t] += nprop;
-        a.ctr[rep_t] = sweep0 + (uint64_t)a.rounds * a.n_sweeps;
-    }
-}
-
 constexpr size_t kG2LdsMax = 64 * 1024;  // two-body matrix staged in LDS up to N = 128
 
 bool exchange_fusable(const pspin_state* st) {
@@ -1079,11 +724,8 @@ using SweepKernel = void (*)(SweepArgs);
 SweepKernel sweep_kernel(const pspin_state* st) {
     const bool bf16 = st->coupling_format == PSPIN_COUPLING_BF16X2;
     if (st->variant == PSPIN_SPARSE) return bf16 ? k_sweep<true, kLists, true> : k_sweep<true, kLists, false>;
-    if (st->kernel == PSPIN_KERNEL_PAIRS) {
-        if (st->pair_reps == 4) return st->pair_rows == 4 ? k_sweep_pairs_k<4, 4> : k_sweep_pairs_k<8, 4>;
-        if (st->pair_reps == 2) return st->pair_rows == 4 ? k_sweep_pairs_k<4, 2> : k_sweep_pairs_k<8, 2>;
+    if (st->kernel == PSPIN_KERNEL_PAIRS)
         return st->pair_rows == 4 ? k_sweep<false, kPairs, false, 4> : k_sweep<false, kPairs, false, 8>;
-    }
     if (st->kernel == PSPIN_KERNEL_GROUPS) {
         if (st->list_batch == 16) return bf16 ? k_sweep<false, kGroups, true, 6> : k_sweep<false, kGroups, false, 6>;
         return bf16 ? k_sweep<false, kGroups, true, 4> : k_sweep<false, kGroups, false, 4>;
@@ -1147,13 +789,10 @@ SweepArgs sweep_args(pspin_state* st, int n_sweeps, bool exchange, int rounds) {
 
 size_t sweep_shm(const SweepArgs& a) { return a.g2_lds ? (size_t)a.N * a.N * sizeof(float) : 0; }
 
-int sweep_grid(const pspin_state* st) {
-    return st->kernel == PSPIN_KERNEL_PAIRS && st->pair_reps > 1 ? st->nrep / st->pair_reps : st->nrep;
-}
 
 hipError_t launch_sweep(pspin_state* st, int n_sweeps, bool exchange) {
     SweepArgs a = sweep_args(st, n_sweeps, exchange, 1);
-    hipLaunchKernelGGL(sweep_kernel(st), dim3(sweep_grid(st)), dim3(sweep_block(st)), sweep_shm(a), st->stream, a);
+    hipLaunchKernelGGL(sweep_kernel(st), dim3(st->nrep), dim3(sweep_block(st)), sweep_shm(a), st->stream, a);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && exchange && !a.exchange) e = launch_exchange(st);
     return e;
@@ -1169,13 +808,13 @@ bool pt_rounds_ok(const pspin_state* st) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sweep_kernel(st), sweep_block(st),
                                                      sweep_shm(a)) != hipSuccess)
         return false;
-    return (long long)per_cu * ncu >= sweep_grid(st);
+    return (long long)per_cu * ncu >= st->nrep;
 }
 
 hipError_t launch_pt_rounds(pspin_state* st, int pt_freq, int rounds) {
     SweepArgs a = sweep_args(st, pt_freq, true, rounds);
     void* args[] = {&a};
-    return hipLaunchCooperativeKernel((const void*)sweep_kernel(st), dim3(sweep_grid(st)), dim3(sweep_block(st)), args,
+    return hipLaunchCooperativeKernel((const void*)sweep_kernel(st), dim3(st->nrep), dim3(sweep_block(st)), args,
                                       (unsigned)sweep_shm(a), st->stream);
 }
 

@@ -267,17 +267,6 @@ int pspin_set_pair_schedule(pspin_state* st, int groups);
  * PSPIN_ERR_ARG when a wave would get more than 64 batches.  pspin_autotune
  * picks it. */
 int pspin_set_pair_tiles(pspin_state* st, int nwaves, int rows);
-/* Replicas per workgroup of the pair-row kernel (1, 2 or 4).  With K > 1 one
- * workgroup serves K replicas of one pair-schedule group and loads each
- * coupling row once for all of them (the tiles stream from L2 at a per-CU
- * rate, so the bytes per pair-update drop K-fold).  Needs a shared pair
- * schedule (pspin_set_pair_schedule) with nrep a multiple of groups x K,
- * and at most 8 waves per workgroup (pspin_set_pair_tiles); changing the
- * schedule so that it no longer fits resets K to 1.  Chains
- * are those of K = 1 up to the FP32 summation order.  pspin_autotune picks
- * it.  No counterpart in the reference (one replica per block,
- * src/mc.cu:784-800). */
-int pspin_set_pair_replicas(pspin_state* st, int replicas_per_workgroup);
 /* Coupling entries in flight per thread in the list kernel (8 or 16; 16 for
  * the dense variant only).  The group kernel keeps 4 (batch 8) or 6 (batch
  * 16) chunks of up to 4 entries in flight.  pspin_autotune picks it. */

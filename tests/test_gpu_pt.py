@@ -129,34 +129,6 @@ def test_launch_modes_give_identical_chains(pspin, oracle, N, NT, nr, kernel):
         assert a[4] == b[4] and a[5] == b[5] == 136
 
 
-def test_replicas_per_workgroup_give_the_same_chains(pspin, oracle):
-    # One, two or four replicas per workgroup of the pair-row kernel, with the
-    # same tile shape: each replica's Delta E sums run in the same order, so
-    # the PT chains agree bit for bit, in cooperative and graph launches.
-    N, NT, nr = 64, 16, 8
-    T, beta = oracle.temperatures(NT, 0.3, 1.6, 1)
-    out = []
-    for reps, mode in ((1, pspin.LAUNCH_AUTO), (2, pspin.LAUNCH_AUTO), (4, pspin.LAUNCH_AUTO), (2, pspin.LAUNCH_GRAPHS),
-                       (4, pspin.LAUNCH_DIRECT)):
-        print(f"[replicas per workgroup] {reps} mode {mode}", flush=True)
-        st = pspin.PSpin(pspin.SimConfig.default(N=N, nrep=NT * nr, seed=77), 0)
-        st.set_kernel(2, 0)
-        st.set_pair_tiles(8, 8)
-        st.set_pair_schedule(8)
-        st.set_pair_replicas(reps)
-        st.pt_init(NT, nr, beta, 77 + 5000)
-        st.set_launch_mode(mode)
-        st.pt_run(64, 1)
-        st.pt_run(30, 3)
-        E, acc, prop = st.results()
-        out.append((st.spins(), E, acc, prop, st.pt_perm(), st.pt_rng48()))
-        st.close()
-    for b in out[1:]:
-        for x, y in zip(out[0][:5], b[:5]):
-            np.testing.assert_array_equal(x, y)
-        assert out[0][5] == b[5]
-
-
 def test_periodic_resync(pspin, oracle):
     cfg = pspin.SimConfig.default(N=14, nrep=4, seed=77)
     st = pspin.PSpin(cfg, 0)

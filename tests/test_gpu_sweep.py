@@ -184,50 +184,6 @@ def test_step_replay_shared_pair_schedule(pspin, oracle, name, kernel, kw, group
     st.close()
 
 
-@pytest.mark.parametrize("name,kw,nrep,groups,tiles,reps", [
-    ("dense FC N=64, 2 replicas per workgroup, 8 groups, 8 waves x 8 rows", dict(N=64), 16, 8, (8, 8), 2),
-    ("dense FC N=64, 4 replicas per workgroup, 4 groups, 4 waves x 4 rows", dict(N=64), 16, 4, (4, 4), 4),
-    ("dense FC N=33 odd, 2 replicas per workgroup, 2 groups", dict(N=33), 16, 2, (8, 8), 2),
-    ("dense FC N=20 J0, 4 replicas per workgroup, 1 wave", dict(N=20, J0=0.4, alpha0=0.5), 16, 4, (1, 4), 4),
-])
-def test_step_replay_replicas_per_workgroup(pspin, oracle, name, kw, nrep, groups, tiles, reps):
-    """K replicas of one pair-schedule group share a workgroup and the coupling
-    loads; each replica's chain is replayed step by step like the default."""
-    print(f"[step replay] {name}", flush=True)
-    cfg = pspin.SimConfig.default(nrep=nrep, seed=4321, fmc_mode=0, **kw)
-    st = pspin.PSpin(cfg, 0)
-    st.set_kernel(2, 0)
-    st.set_pair_tiles(*tiles)
-    st.set_pair_schedule(groups)
-    st.set_pair_replicas(reps)
-    betas = np.linspace(0.5, 3.0, nrep)
-    stats = replay_check(oracle, st, 2, betas)
-    print(f"  max |dE32-dE64|/S = {stats['worst_scaled']:.2e}  acc={stats['acc_rate']:.3f}", flush=True)
-    st.close()
-
-
-def test_replicas_per_workgroup_argument_checks(pspin):
-    st = pspin.PSpin(pspin.SimConfig.default(N=64, nrep=12, seed=1, fmc_mode=0), 0)
-    st.set_kernel(2, 0)
-    with pytest.raises(pspin.PSpinError):
-        st.set_pair_replicas(2)  # no pair schedule
-    st.set_pair_schedule(6)
-    with pytest.raises(pspin.PSpinError):
-        st.set_pair_replicas(4)  # 12 is not a multiple of 6 x 4
-    st.set_pair_schedule(3)
-    st.set_pair_tiles(16, 8)
-    with pytest.raises(pspin.PSpinError):
-        st.set_pair_replicas(2)  # more than 8 waves
-    st.set_pair_tiles(8, 8)
-    st.set_pair_replicas(2)
-    with pytest.raises(pspin.PSpinError):
-        st.set_pair_tiles(12, 8)
-    st.set_pair_schedule(0)  # resets to one replica per workgroup
-    st.set_uniform_beta(1.0)
-    st.sweep(2)
-    st.close()
-
-
 def test_block_sizes_agree(pspin, oracle):
     # Same chain at different workgroup sizes: identical pairs and proposals,
     # Delta E within tolerance (different reduction order), decisions equal.
