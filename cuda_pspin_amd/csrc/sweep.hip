@@ -527,8 +527,10 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             // one round trip after the barrier; the partials, sites and
             // threshold by lane gathers taken with readlane, the amplitudes
             // as uniform-address vector reads.  The proposal picked by the
-            // decision is a second read.  Every wave sums the partials in the
-            // same fixed order and so takes the same decision.
+            // decision is a second read (cheaper in issue slots, which the four
+            // waves of a SIMD share, than gathering both hypotheses).  Every
+            // wave sums the partials in the same fixed order and so takes the
+            // same decision.
             const bool more = step < nsteps;
             const int sn = more ? step : 0;
             const int sp = step > 0 ? step - 1 : 0;  // the step resolved here (none at step 0)
