@@ -998,8 +998,9 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
     PS_REQUIRE(st && beta_sched, PSPIN_ERR_ARG, "null argument");
     PS_REQUIRE(NT >= 2 && nrep_per_temp >= 1 && (long long)NT * nrep_per_temp == st->nrep, PSPIN_ERR_ARG,
                "NT * nrep_per_temp must equal nrep and NT >= 2");
-    PS_REQUIRE(ps::exchange_lds_bytes(NT, nrep_per_temp) <= 60 * 1024, PSPIN_ERR_ARG,
-               "too many replicas for the exchange kernel (NT * nrep_per_temp <= 3000)");
+    PS_REQUIRE(ps::exchange_kernel_lds_bytes(NT, nrep_per_temp) <= 60 * 1024, PSPIN_ERR_ARG,
+               "too many replicas for the exchange kernel (NT * nrep_per_temp <= about 2100 with up to 64 "
+               "ladders, 5000 with more)");
     drop_graph(st);
     dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48); dfree(st->d_done);
     st->pt_NT = NT;

@@ -133,8 +133,7 @@ hipError_t launch_exchange(pspin_state* st) {
     if (a.nrep <= 64) {
         hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), exchange_lds_bytes(a.NT, a.nrep), st->stream, a);
     } else {
-        size_t lds = (size_t)a.NT * a.nrep * (sizeof(double) + sizeof(int)) + (size_t)a.NT * (sizeof(double) + 8);
-        hipLaunchKernelGGL(k_pt_exchange_wide, dim3(1), dim3(64), lds, st->stream, a);
+        hipLaunchKernelGGL(k_pt_exchange_wide, dim3(1), dim3(64), exchange_wide_lds_bytes(a.NT, a.nrep), st->stream, a);
     }
     return hipGetLastError();
 }

@@ -47,26 +47,40 @@ __device__ __forceinline__ void dev_store(T* p, T v) {
 }
 
 // LDS of the one-wave exchange (n = NT nrep slots): sE[n] energies by slot,
-// sB[NT] betas, sacc[NT] accepted swaps per level, sX[n] the lcg48 states of
-// every draw the round can take, sP[n] replicas by slot.
+// sEr[n] energies by replica (as loaded), sB[NT] betas, sacc[NT] accepted
+// swaps per level, sctr[4 NT] the exchange counters as loaded, sX[n] the
+// lcg48 states of every draw the round can take, sX0 the 48-bit state and
+// sser the series counter as loaded, sP[n] replicas by slot.
 struct ExLds {
     double* sE;
+    double* sEr;
     double* sB;
     long long* sacc;
+    long long* sctr;
     uint64_t* sX;
+    uint64_t* sX0;
+    long long* sser;
     int* sP;
     __device__ ExLds(void* lds, int NT, int n) {
         sE = (double*)lds;
-        sB = sE + n;
+        sEr = sE + n;
+        sB = sEr + n;
         sacc = (long long*)(sB + NT);
-        sX = (uint64_t*)(sacc + NT);
-        sP = (int*)(sX + n);
+        sctr = sacc + NT;
+        sX = (uint64_t*)(sctr + 4 * NT);
+        sX0 = sX + n;
+        sser = (long long*)(sX0 + 1);
+        sP = (int*)(sser + 1);
     }
 };
 
-// exchange_stage fills the slots with nthr threads (loads first, then LDS
-// writes); the caller then orders LDS (barrier or wave fence) before
-// exchange_wave.
+// exchange_stage fills the slots with nthr threads.  Everything the round
+// reads from global memory is requested at once, in one round trip: the
+// permutation, every replica's energy (by replica, so no load waits for the
+// permutation), the betas, the exchange counters, the 48-bit state and the
+// series counter.  The energies are then put in slot order in LDS.  It
+// contains a workgroup barrier, so all nthr threads call it; the caller then
+// orders LDS (barrier or wave fence) before exchange_wave.
 __device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int nthr) {
     asm volatile("" : "+v"(tid));  // nothing thread-derived hoisted out of a caller's loop
     const int NT = a.NT, n = NT * a.nrep;
@@ -75,17 +89,26 @@ __device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int n
         int p[4];
         double e[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) p[k] = i0 + k * nthr < n ? dev_load(a.perm + i0 + k * nthr) : 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) e[k] = i0 + k * nthr < n ? dev_load(a.E + p[k]) : 0.0;
+        for (int k = 0; k < 4; k++) {
+            const int i = i0 + k * nthr;
+            p[k] = i < n ? dev_load(a.perm + i) : 0;
+            e[k] = i < n ? dev_load(a.E + i) : 0.0;
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++)
             if (i0 + k * nthr < n) {
                 L.sP[i0 + k * nthr] = p[k];
-                L.sE[i0 + k * nthr] = e[k];
+                L.sEr[i0 + k * nthr] = e[k];
             }
     }
     for (int i = tid; i < NT; i += nthr) L.sB[i] = a.beta_sched[i];
+    for (int i = tid; i < 4 * (NT - 1); i += nthr) L.sctr[i] = dev_load(a.ex + i);
+    if (tid == 0) {
+        *L.sX0 = dev_load(a.rng48);
+        *L.sser = a.series ? dev_load(a.series_n) : 0;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nthr) L.sE[i] = L.sEr[L.sP[i]];
 }
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
@@ -106,7 +129,7 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
     const int NT = a.NT, nr = a.nrep, n = NT * nr;
     const bool valid = lane < nr;
     ExLds L(lds, NT, n);
-    const uint64_t X0 = dev_load(a.rng48);
+    const uint64_t X0 = *L.sX0;
     {
         // X_{k+1} for k = 0 .. nr (NT - 1) - 1: lane l jumps l + 1 steps, then
         // 64 steps at a time (lane 63's coefficients).
@@ -164,23 +187,21 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
         dev_store(a.perm + i, p);
         dev_store(a.betas + p, L.sB[i / nr]);
     }
-    for (int t = lane; t < NT - 1; t += 64) {  // all four loads before any store
+    for (int t = lane; t < NT - 1; t += 64) {  // the counters as staged, plus this round
         long long* ex = a.ex;
-        const long long c0 = dev_load(ex + t), c1 = dev_load(ex + (NT - 1) + t);
-        const long long c2 = dev_load(ex + 2 * (NT - 1) + t), c3 = dev_load(ex + 3 * (NT - 1) + t);
         const long long k = L.sacc[t];
-        dev_store(ex + t, c0 + k);
-        dev_store(ex + (NT - 1) + t, c1 + nr);
-        dev_store(ex + 2 * (NT - 1) + t, c2 + k);
-        dev_store(ex + 3 * (NT - 1) + t, c3 + nr);
+        dev_store(ex + t, L.sctr[t] + k);
+        dev_store(ex + (NT - 1) + t, L.sctr[(NT - 1) + t] + nr);
+        dev_store(ex + 2 * (NT - 1) + t, L.sctr[2 * (NT - 1) + t] + k);
+        dev_store(ex + 3 * (NT - 1) + t, L.sctr[3 * (NT - 1) + t] + nr);
     }
     if (a.series) {
-        const long long round = dev_load(a.series_n);
+        const long long round = *L.sser;
         if (valid) dev_store(a.series + (size_t)(round % a.series_cap) * nr + lane, ec);
     }
     if (lane == 0) {
         dev_store(a.rng48, base > 0 ? L.sX[base - 1] : X0);
-        if (a.series) dev_store(a.series_n, dev_load(a.series_n) + 1);
+        if (a.series) dev_store(a.series_n, *L.sser + 1);
     }
 }
 

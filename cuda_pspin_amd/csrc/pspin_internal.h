@@ -298,9 +298,21 @@ struct ExArgs {
     int series_cap;
 };
 ExArgs exchange_args(const pspin_state* st);
-// LDS bytes of the one-wave exchange for NT x nrep slots.
+// LDS bytes of the one-wave exchange for NT x nrep slots (layout in ExLds,
+// exchange_dev.h): energies by slot and by replica, draw states and the
+// permutation per slot; betas, accept counts and the four exchange counters
+// per level; the 48-bit state and the series counter.
 inline size_t exchange_lds_bytes(int NT, int nrep) {
-    return (size_t)NT * nrep * (2 * sizeof(double) + sizeof(int)) + (size_t)NT * 2 * sizeof(double);
+    return (size_t)NT * nrep * (3 * sizeof(double) + sizeof(int)) + (size_t)NT * 6 * sizeof(double) +
+           2 * sizeof(double);
+}
+// LDS bytes of the many-ladder exchange kernel (more than 64 ladders).
+inline size_t exchange_wide_lds_bytes(int NT, int nrep) {
+    return (size_t)NT * nrep * (sizeof(double) + sizeof(int)) + (size_t)NT * (sizeof(double) + 8);
+}
+// LDS bytes of the exchange kernel that serves NT x nrep slots.
+inline size_t exchange_kernel_lds_bytes(int NT, int nrep) {
+    return nrep <= 64 ? exchange_lds_bytes(NT, nrep) : exchange_wide_lds_bytes(NT, nrep);
 }
 // Whether a sweep launch can run the exchange in its last workgroup.
 bool exchange_fusable(const pspin_state* st);
