@@ -19,7 +19,7 @@
 // that contain both sites, and in differential form, so FP32 arithmetic does
 // not cancel new-minus-old products.
 //
-// Two coupling layouts feed the same step machinery:
+// Three coupling layouts feed the same step machinery:
 //  LISTS  site x owns a contiguous list of (p | m<<8 | n<<16, g) entries
 //         (any channel pattern: FMC, comb, sparse quartets); coalesced 8-byte
 //         loads, the three amplitudes from LDS.
@@ -36,6 +36,10 @@
 //         (partner c) with U_bc a product of two amplitudes, from a per-site
 //         table of LDS offsets and partner flags.  A static list of row
 //         segments balances the waves.
+//  GROUPS dense, any channel pattern (FMC, comb): site x's entries grouped by
+//         their conjugated pair {m, n} in chunks of up to four partners, so a
+//         chunk adds conj(s~_m s~_n) sum_k g_k s~_pk to the field of x: one
+//         complex product per chunk, one LDS read and 2 FMA per entry.
 //
 // Randomness: Philox4x32-10 keyed by the replica seed (the reference's
 // splitmix64 chain, src/mc.cu:418-423) and countered by (sweep, step), all N/2
@@ -297,9 +301,10 @@ __device__ __forceinline__ float pair_step(const SweepArgs& a, PairBuf<PB>& B, i
 // conj(s~_m s~_n) * sum_k g_k s~_pk to the field of x: 3 LDS reads and 4 FMA
 // per chunk plus 1 LDS read and 2 FMA per entry, against 3 reads and about
 // 10 FMA per entry in the LISTS layout.  The chunk ranges of the step's two
-// sites are laid end to end; a thread takes chunks tid, tid + bdim, ... GB at
-// a time.  Loads run one batch ahead of the arithmetic, and the last batch of
-// a step loads the next step's first batch, so those loads cross the barrier.
+// sites are laid end to end; thread t takes chunks (bdim - 1 - t) + u bdim,
+// GB at a time.  Loads run one batch ahead of the arithmetic, and the last
+// batch of a step loads the next step's first batch, so those loads cross the
+// barrier.
 template <int GB>
 struct GrpBuf {
     uint2 h[GB];
