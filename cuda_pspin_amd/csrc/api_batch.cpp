@@ -44,8 +44,10 @@ struct pspin_batch {
     long long *d_acc = nullptr, *d_prop = nullptr, *d_ex = nullptr, *d_done = nullptr;
     uint64_t *d_seeds = nullptr, *d_ctr = nullptr, *d_rng48 = nullptr;
     int *d_since = nullptr, *d_perm = nullptr;
+    ps::BatchDraw* d_draws = nullptr;  // per-sweep draw scratch
+    int max_steps = 1;
     int resync_every = 1024;
-    int threads = 64;
+    int chains_per_sample = 1;  // NT * nrep_per_temp (picks the lanes per chain)
     size_t lds = 0;
     long long pair_updates_per_sweep = 0;
     hipGraphExec_t graph = nullptr;
@@ -76,7 +78,7 @@ void batch_free(pspin_batch* b) {
     dfree(b->d_samples); dfree(b->d_e4); dfree(b->d_e2); dfree(b->d_o4); dfree(b->d_o2); dfree(b->d_q);
     dfree(b->d_qg); dfree(b->d_h2g); dfree(b->d_h2ij); dfree(b->d_spins); dfree(b->d_E); dfree(b->d_betas);
     dfree(b->d_sched); dfree(b->d_acc); dfree(b->d_prop); dfree(b->d_ex); dfree(b->d_done); dfree(b->d_seeds);
-    dfree(b->d_ctr); dfree(b->d_rng48); dfree(b->d_since); dfree(b->d_perm);
+    dfree(b->d_ctr); dfree(b->d_rng48); dfree(b->d_since); dfree(b->d_perm); dfree(b->d_draws);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -103,6 +105,7 @@ int batch_args(pspin_batch* b, ps::BatchArgs& a, int n_sweeps, int pt_freq) {
     a.spins = b->d_spins; a.E = b->d_E; a.acc = b->d_acc; a.prop = b->d_prop; a.seeds = b->d_seeds;
     a.ctr = b->d_ctr; a.since = b->d_since; a.betas = b->d_betas; a.perm = b->d_perm;
     a.beta_sched = b->d_sched; a.ex = b->d_ex; a.rng48 = b->d_rng48; a.done = b->d_done;
+    a.draws = b->d_draws; a.max_steps = b->max_steps;
     return PSPIN_OK;
 }
 
@@ -115,7 +118,7 @@ int pspin_batch_create(const pspin_sim_config* base, int n_samples, const int* N
     PB_REQUIRE(base && Ns && seeds && beta_sched && out, PSPIN_ERR_ARG, "null argument");
     PB_REQUIRE(n_samples >= 1 && NT >= 2 && nrep_per_temp >= 1, PSPIN_ERR_ARG, "need n_samples >= 1, NT >= 2");
     const int C = NT * nrep_per_temp;
-    PB_REQUIRE(C <= 256, PSPIN_ERR_ARG, "NT * nrep_per_temp must be <= 256 (one lane per chain)");
+    PB_REQUIRE(C <= 256, PSPIN_ERR_ARG, "NT * nrep_per_temp must be <= 256 (4 lanes per chain in a 1024-thread workgroup)");
     *out = nullptr;
     pspin_batch* b = new pspin_batch();
     b->n = n_samples;
@@ -232,7 +235,9 @@ int pspin_batch_create(const pspin_sim_config* base, int n_samples, const int* N
     }
     PB_REQUIRE(lds_max <= 160 * 1024, PSPIN_ERR_ARG, "a sample needs more than 160 KB of LDS (N * chains too large)");
     b->lds = lds_max;
-    b->threads = (C + 63) / 64 * 64;
+    // the draws of a sweep are made by every thread; each chain runs on 4 or 8 lanes
+    b->chains_per_sample = C;
+    for (const ps::BatchSample& S : b->h_samples) b->max_steps = std::max(b->max_steps, S.N / 2);
     hipStream_t s = b->stream;
     const size_t nch = (size_t)b->chains;
     std::vector<uint64_t> zero64(nch, 0);
@@ -262,6 +267,7 @@ int pspin_batch_create(const pspin_sim_config* base, int n_samples, const int* N
     PB_ALLOC(b->d_rng48, rng48.size(), rng48.data());
     PB_ALLOC(b->d_since, nch, zero32.data());
     PB_ALLOC(b->d_perm, perm.size(), perm.data());
+    PB_ALLOC(b->d_draws, nch * (size_t)b->max_steps, nullptr);
 #undef PB_ALLOC
     b->n_spin = spins.size();
     b->n_perm = perm.size();
@@ -314,7 +320,7 @@ int batch_advance(pspin_batch* b, int n_sweeps, int pt_freq) {
                 PB_TRY_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
                 hipError_t e = hipSuccess;
                 for (int k = 0; k < launches && e == hipSuccess; k++)
-                    e = ps::launch_batch(a, b->n, b->threads, b->lds, b->stream);
+                    e = ps::launch_batch(a, b->n, b->chains_per_sample, b->lds, b->stream);
                 hipError_t e2 = hipStreamEndCapture(b->stream, &g);
                 PB_TRY_HIP(e);
                 PB_TRY_HIP(e2);
@@ -330,7 +336,7 @@ int batch_advance(pspin_batch* b, int n_sweeps, int pt_freq) {
         }
         const int c = std::min(left, chunk);
         batch_args(b, a, c, pt_freq);
-        PB_TRY_HIP(ps::launch_batch(a, b->n, b->threads, b->lds, b->stream));
+        PB_TRY_HIP(ps::launch_batch(a, b->n, b->chains_per_sample, b->lds, b->stream));
         done += c;
     }
     return PSPIN_OK;

@@ -13,6 +13,16 @@
 // with the reference's decision order and drand48 stream for that sample,
 // and every chain picks up its new beta.
 //
+// A sweep runs in two phases.  First every thread of the workgroup makes the
+// state-independent draws of all (step, chain) pairs of the sample (Philox
+// blocks, pair, phases, intensity split, -log u of the threshold) into a
+// global scratch array, [step][chain]; these FP64 transcendentals are most of
+// a step's instructions, and this way they run with the whole workgroup's
+// waves hiding their latency.  Then each chain's lane walks its steps,
+// prefetching the next step's draws, with only the amplitude-dependent part
+// left: three square roots of the cube proposal, the local fields and the
+// decision.
+//
 // Per chain the arithmetic is the dense kernel's: FP64 cube proposal
 // (include/spins_sparse.h:26-56), FP32 midpoint local-field Delta E
 // (exact for the multilinear H, see sweep.hip), threshold Metropolis, FP64
@@ -27,16 +37,64 @@ namespace ps {
 
 __device__ __forceinline__ float2 f2(double2 v) { return make_float2((float)v.x, (float)v.y); }
 
-// LDS layout (dynamic): FP64 amplitudes [N][C] (site-major so lanes of one
-// site are adjacent), H4 entries, H2 entries, offsets, exchange scratch.
-__global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
+// The draws of one step of one chain (k_sweep's make_step with the cube
+// proposal's amplitude-independent factors; src/mc_sparse.cu:118-126,
+// include/spins_sparse.h:34-41).
+__device__ __forceinline__ BatchDraw batch_draw(uint64_t seed, uint64_t sweep, int step, int N) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep0}, k0, k1);
+    const U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep1}, k0, k1);
+    int ri = (int)(u32_to_unit(x.x) * N);
+    if (ri >= N) ri = N - 1;
+    int rj = (int)(u32_to_unit(x.y) * (N - 1));
+    if (rj >= N - 1) rj = N - 2;
+    if (rj >= ri) rj++;
+    BatchDraw d;
+    d.ij = (ri < rj ? ri : rj) | ((ri < rj ? rj : ri) << 16);
+    d.pad = 0;
+    const double u = u32_to_unit(x.z);
+    sincos(2.0 * M_PI * u32_to_unit(x.w), &d.s1, &d.c1);
+    sincos(2.0 * M_PI * u32_to_unit(y.x), &d.s2, &d.c2);
+    d.q1 = sqrt(1.0 - u);
+    d.q2 = sqrt(u);
+    d.lu = -log(u32_to_unit(y.y));
+    return d;
+}
+
+// Sum over the L lanes of one chain (L = 4 or 8 consecutive lanes) with DPP
+// moves.  Each stage adds a lane's value to its partner's, and a + b == b + a
+// exactly, so every lane of the chain ends with the same bits.
+template <int L>
+__device__ __forceinline__ float chain_sum(float v) {
+    static_assert(L == 4 || L == 8, "4 or 8 lanes per chain");
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xb1, 0xf, 0xf, false));  // quad_perm 1,0,3,2
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4e, 0xf, 0xf, false));  // quad_perm 2,3,0,1
+    if (L == 8)
+        v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xf, 0xf, false));  // row_half_mirror
+    return v;
+}
+
+// LDS layout (dynamic): FP64 amplitudes [N][P] with P = C | 1 (site-major so
+// the chains of one site are adjacent; the odd pitch puts the different sites
+// that the lanes of one chain read in different banks), H4 entries, H2
+// entries, offsets, exchange scratch.
+//
+// Lanes: L consecutive lanes own one chain c = tid / L.  All of them make the
+// chain's proposal and decision (same inputs, same bits); lanes [0, L/2) sum
+// the fields of site i over every (L/2)-th entry of its lists, lanes [L/2, L)
+// those of site j, and chain_sum adds the L partial Delta E.  Lane sub = 0
+// writes the accepted amplitudes and owns the chain's counters and energy.
+template <int L>
+__global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
     extern __shared__ double lds[];
     const BatchSample S = a.samples[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63;
-    const int N = S.N, C = S.chains, NT = a.NT, nr = C / NT;
-    const bool live = tid < C;
-    double2* amp = (double2*)lds;                       // [N][C]
-    uint2* e4 = (uint2*)(amp + (size_t)N * C);          // H4 entries
+    const int N = S.N, C = S.chains, NT = a.NT, nr = C / NT, P = C | 1;
+    constexpr int H = L / 2;
+    const int c = tid / L, sub = tid % L, half = sub / H, hs = sub % H;
+    const bool live = c < C, lead = live && sub == 0;
+    double2* amp = (double2*)lds;                       // [N][P]
+    uint2* e4 = (uint2*)(amp + (size_t)N * P);          // H4 entries
     uint2* e2 = e4 + S.n_e4;                            // H2 entries {k, g}
     int* o4 = (int*)(e2 + S.n_e2);                      // [N+1]
     int* o2 = o4 + N + 1;                               // [N+1]
@@ -54,21 +112,20 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
         o4[i] = a.o4[S.o_off + i];
         o2[i] = a.o2[S.o_off + i];
     }
-    if (live)
-        for (int x = 0; x < N; x++) {
-            const double* g = a.spins + S.spin_off + ((size_t)tid * N + x) * 2;
-            amp[x * C + tid] = make_double2(g[0], g[1]);
-        }
+    for (int x = tid; x < N * C; x += blockDim.x) {
+        const int cc = x / N, site = x - cc * N;
+        const double* g = a.spins + S.spin_off + (size_t)x * 2;
+        amp[site * P + cc] = make_double2(g[0], g[1]);
+    }
     if (tid < C) sperm[tid] = a.perm[S.chain_off + tid];
     if (tid < 64) lcg48_coeffs((uint64_t)tid + 1, &jA[tid + 1], &jC[tid + 1]);
     if (tid == 0) {
         jA[0] = 1;
         jC[0] = 0;
     }
-    const int ch = S.chain_off + tid;
+    const int ch = S.chain_off + c;
     double beta = live ? a.betas[ch] : 0.0;
     double E = live ? a.E[ch] : 0.0;
-    const uint64_t seed = live ? a.seeds[ch] : 0;
     uint64_t sweep = live ? a.ctr[ch] : 0;
     long long nacc = 0, nprop = 0;
     int since = live ? a.since[ch] : 0;
@@ -77,66 +134,68 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
     __syncthreads();
 
     const int nsteps = N / 2;
+    BatchDraw* dr = a.draws + (size_t)S.chain_off * a.max_steps;  // this sample's [step][chain] draws
     for (int sw = 0; sw < a.n_sweeps; sw++) {
+        // Phase 1: every (step, chain) draw of the sweep, all threads (after the
+        // chains have read the previous sweep's draws).
+        if (sw > 0) __syncthreads();
+        for (int x = tid; x < nsteps * C; x += blockDim.x) {
+            const int st = x / C, cx = x - st * C;
+            const int gc = S.chain_off + cx;
+            dr[x] = batch_draw(a.seeds[gc], a.ctr[gc] + (uint64_t)sw, st, N);
+        }
+        __syncthreads();
+        // Phase 2: each chain's steps, the next step's draws loaded one ahead.
         if (live) {
-            const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+            BatchDraw nd = dr[c];
             for (int step = 0; step < nsteps; step++) {
-                const U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep0}, k0, k1);
-                const U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep1}, k0, k1);
-                // pair pick (src/mc_sparse.cu:118-126)
-                int ri = (int)(u32_to_unit(x.x) * N);
-                if (ri >= N) ri = N - 1;
-                int rj = (int)(u32_to_unit(x.y) * (N - 1));
-                if (rj >= N - 1) rj = N - 2;
-                if (rj >= ri) rj++;
-                const int i0 = ri < rj ? ri : rj, j0 = ri < rj ? rj : ri;
+                const BatchDraw d = nd;
+                if (step + 1 < nsteps) nd = dr[(size_t)(step + 1) * C + c];
+                const int i0 = d.ij & 0xffff, j0 = d.ij >> 16;
                 // cube proposal (include/spins_sparse.h:26-56)
-                const double2 ai = amp[i0 * C + tid], aj = amp[j0 * C + tid];
-                const double u = u32_to_unit(x.z);
-                double s1, c1, s2, c2;
-                sincos(2.0 * M_PI * u32_to_unit(x.w), &s1, &c1);
-                sincos(2.0 * M_PI * u32_to_unit(y.x), &s2, &c2);
+                const double2 ai = amp[i0 * P + c], aj = amp[j0 * P + c];
                 const double r1 = ai.x * ai.x + ai.y * ai.y, r2 = aj.x * aj.x + aj.y * aj.y;
                 const double sq = sqrt(r1 * r1 + r2 * r2);
-                const double m1 = sqrt(sq * sqrt(1.0 - u)), m2 = sqrt(sq * sqrt(u));
-                const double2 ni = make_double2(m1 * c1, m1 * s1), nj = make_double2(m2 * c2, m2 * s2);
-                const float2 di = make_float2((float)(ni.x - ai.x), (float)(ni.y - ai.y));
-                const float2 dj = make_float2((float)(nj.x - aj.x), (float)(nj.y - aj.y));
-                const float2 mi = make_float2((float)(0.5 * (ai.x + ni.x)), (float)(0.5 * (ai.y + ni.y)));
-                const float2 mj = make_float2((float)(0.5 * (aj.x + nj.x)), (float)(0.5 * (aj.y + nj.y)));
-                // local fields at the midpoint configuration
-                float dE = 0.f;
-                for (int pass = 0; pass < 2; pass++) {
-                    const int site = pass ? j0 : i0, other = pass ? i0 : j0;
-                    const float2 mo = pass ? mi : mj;
-                    float2 F = make_float2(0.f, 0.f);
-                    for (int e = o2[site]; e < o2[site + 1]; e++) {
-                        const uint2 en = e2[e];
-                        const int k = (int)en.x;
-                        const float g = __uint_as_float(en.y);
-                        const float2 t = (k == other) ? mo : f2(amp[k * C + tid]);
-                        F.x += g * t.x;
-                        F.y -= g * t.y;
-                    }
-                    for (int e = o4[site]; e < o4[site + 1]; e++) {
-                        const uint2 en = e4[e];
-                        const int p = en.x & 0xff, m = (en.x >> 8) & 0xff, n = (en.x >> 16) & 0xff;
-                        const float g = __uint_as_float(en.y);
-                        const float2 tp = (p == other) ? mo : f2(amp[p * C + tid]);
-                        const float2 tm = (m == other) ? mo : f2(amp[m * C + tid]);
-                        const float2 tn = (n == other) ? mo : f2(amp[n * C + tid]);
-                        const float2 mn = make_float2(tm.x * tn.x - tm.y * tn.y, tm.x * tn.y + tm.y * tn.x);
-                        F.x += g * (tp.x * mn.x + tp.y * mn.y);
-                        F.y += g * (tp.y * mn.x - tp.x * mn.y);
-                    }
-                    const float2 d = pass ? dj : di;
-                    dE -= d.x * F.x - d.y * F.y;
+                const double m1 = sqrt(sq * d.q1), m2 = sqrt(sq * d.q2);
+                const double2 ni = make_double2(m1 * d.c1, m1 * d.s1), nj = make_double2(m2 * d.c2, m2 * d.s2);
+                // this lane's site (i for the first half of the chain's lanes, j for the second)
+                const int site = half ? j0 : i0, other = half ? i0 : j0;
+                const double2 as = half ? aj : ai, ns = half ? nj : ni;
+                const double2 ao = half ? ai : aj, no = half ? ni : nj;
+                const float2 dd = make_float2((float)(ns.x - as.x), (float)(ns.y - as.y));
+                const float2 mo = make_float2((float)(0.5 * (ao.x + no.x)), (float)(0.5 * (ao.y + no.y)));
+                // local field at the midpoint configuration, this lane's share of the entries
+                float2 F = make_float2(0.f, 0.f);
+                for (int e = o2[site] + hs; e < o2[site + 1]; e += H) {
+                    const uint2 en = e2[e];
+                    const int k = (int)en.x;
+                    const float g = __uint_as_float(en.y);
+                    const float2 t = (k == other) ? mo : f2(amp[k * P + c]);
+                    F.x += g * t.x;
+                    F.y -= g * t.y;
                 }
-                const double thr = (beta == 0.0) ? INFINITY : -log(u32_to_unit(y.y)) / beta;
+                for (int e = o4[site] + hs; e < o4[site + 1]; e += H) {
+                    const uint2 en = e4[e];
+                    const int p = en.x & 0xff, m = (en.x >> 8) & 0xff, n = (en.x >> 16) & 0xff;
+                    const float g = __uint_as_float(en.y);
+                    const float2 tp = (p == other) ? mo : f2(amp[p * P + c]);
+                    const float2 tm = (m == other) ? mo : f2(amp[m * P + c]);
+                    const float2 tn = (n == other) ? mo : f2(amp[n * P + c]);
+                    const float2 mn = make_float2(tm.x * tn.x - tm.y * tn.y, tm.x * tn.y + tm.y * tn.x);
+                    F.x += g * (tp.x * mn.x + tp.y * mn.y);
+                    F.y += g * (tp.y * mn.x - tp.x * mn.y);
+                }
+                const float dE = chain_sum<L>(-(dd.x * F.x - dd.y * F.y));
+                const double thr = (beta == 0.0) ? INFINITY : d.lu / beta;
                 nprop++;
                 if (beta == 0.0 || (double)dE < thr) {
-                    amp[i0 * C + tid] = ni;
-                    amp[j0 * C + tid] = nj;
+                    if (sub == 0) {
+                        amp[i0 * P + c] = ni;
+                        amp[j0 * P + c] = nj;
+                    }
+                    // the chain's other lanes read these next step from the same
+                    // wave: keep every later LDS access after the stores
+                    __builtin_amdgcn_wave_barrier();
                     E += (double)dE;
                     nacc++;
                 }
@@ -145,32 +204,34 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
             since++;
             if (a.resync_every > 0 && since >= a.resync_every) {
                 // FP64 resync from the sample's FP64 terms (src/mc_sparse.cu:258-315, i<j H2)
-                double e = 0.0;
-                for (int t = 0; t < S.n_q; t++) {
-                    const int* v = a.quartets + (size_t)5 * (S.q_off + t);
-                    const double2 A0 = amp[v[0] * C + tid], A1 = amp[v[1] * C + tid];
-                    const double2 A2 = amp[v[2] * C + tid], A3 = amp[v[3] * C + tid];
-                    const int cm = conj_mask(v[4]);
-                    const double2 s1_ = (cm & 2) ? make_double2(A1.x, -A1.y) : A1;
-                    const double2 s2_ = (cm & 4) ? make_double2(A2.x, -A2.y) : A2;
-                    const double2 s3_ = (cm & 8) ? make_double2(A3.x, -A3.y) : A3;
-                    const double2 p01 = make_double2(A0.x * s1_.x - A0.y * s1_.y, A0.x * s1_.y + A0.y * s1_.x);
-                    const double2 p23 = make_double2(s2_.x * s3_.x - s2_.y * s3_.y, s2_.x * s3_.y + s2_.y * s3_.x);
-                    e += -a.qg[S.q_off + t] * (p01.x * p23.x - p01.y * p23.y);
+                if (sub == 0) {
+                    double e = 0.0;
+                    for (int t = 0; t < S.n_q; t++) {
+                        const int* v = a.quartets + (size_t)5 * (S.q_off + t);
+                        const double2 A0 = amp[v[0] * P + c], A1 = amp[v[1] * P + c];
+                        const double2 A2 = amp[v[2] * P + c], A3 = amp[v[3] * P + c];
+                        const int cm = conj_mask(v[4]);
+                        const double2 s1_ = (cm & 2) ? make_double2(A1.x, -A1.y) : A1;
+                        const double2 s2_ = (cm & 4) ? make_double2(A2.x, -A2.y) : A2;
+                        const double2 s3_ = (cm & 8) ? make_double2(A3.x, -A3.y) : A3;
+                        const double2 p01 = make_double2(A0.x * s1_.x - A0.y * s1_.y, A0.x * s1_.y + A0.y * s1_.x);
+                        const double2 p23 = make_double2(s2_.x * s3_.x - s2_.y * s3_.y, s2_.x * s3_.y + s2_.y * s3_.x);
+                        e += -a.qg[S.q_off + t] * (p01.x * p23.x - p01.y * p23.y);
+                    }
+                    for (int t = 0; t < S.n_h2; t++) {
+                        const int2 ij = a.h2ij[S.h2_off + t];
+                        const double2 A = amp[ij.x * P + c], B = amp[ij.y * P + c];
+                        e += -a.h2g[S.h2_off + t] * (A.x * B.x + A.y * B.y);
+                    }
+                    E = e;
                 }
-                for (int t = 0; t < S.n_h2; t++) {
-                    const int2 ij = a.h2ij[S.h2_off + t];
-                    const double2 A = amp[ij.x * C + tid], B = amp[ij.y * C + tid];
-                    e += -a.h2g[S.h2_off + t] * (A.x * B.x + A.y * B.y);
-                }
-                E = e;
                 since = 0;
             }
         }
         const bool exch = a.pt_freq > 0 && (done0 + sw + 1) % a.pt_freq == 0;
         if (exch) {
             // Exchange (src/parallel_tempering.cu:316-352) for this sample.
-            if (live) sE[tid] = E;
+            if (lead) sE[c] = E;
             __syncthreads();
             if (tid < 64) {
                 const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -210,46 +271,56 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
                 for (int i = lane; i < C; i += 64) sB[sperm[i]] = a.beta_sched[i / nr];
             }
             __syncthreads();
-            if (live) beta = sB[tid];
+            if (live) beta = sB[c];
         }
     }
 
-    // ---- write back
-    if (live) {
-        for (int x = 0; x < N; x++) {
-            double* g = a.spins + S.spin_off + ((size_t)tid * N + x) * 2;
-            const double2 v = amp[x * C + tid];
-            g[0] = v.x;
-            g[1] = v.y;
-        }
+    // ---- write back (amplitudes were last written by other waves' lead lanes)
+    __syncthreads();
+    for (int x = tid; x < N * C; x += blockDim.x) {
+        const int cc = x / N, site = x - cc * N;
+        double* g = a.spins + S.spin_off + (size_t)x * 2;
+        const double2 v = amp[site * P + cc];
+        g[0] = v.x;
+        g[1] = v.y;
+    }
+    if (lead) {
         a.E[ch] = E;
         a.acc[ch] += nacc;
         a.prop[ch] += nprop;
         a.ctr[ch] = sweep;
         a.since[ch] = since;
         a.betas[ch] = beta;
-        a.perm[S.chain_off + tid] = sperm[tid];
     }
+    if (tid < C) a.perm[S.chain_off + tid] = sperm[tid];
     if (tid == 0) {
         a.rng48[blockIdx.x] = X;
         a.done[blockIdx.x] = done0 + a.n_sweeps;
     }
 }
 
+int batch_lanes(int chains) { return chains <= kBatchThreads / 8 ? 8 : 4; }
+
 size_t batch_lds_bytes(const BatchSample& s, int NT) {
-    size_t b = (size_t)s.N * s.chains * 16 + (size_t)(s.n_e4 + s.n_e2) * 8 + (size_t)2 * (s.N + 1) * 4;
+    size_t b = (size_t)s.N * (s.chains | 1) * 16 + (size_t)(s.n_e4 + s.n_e2) * 8 + (size_t)2 * (s.N + 1) * 4;
     b = (b + 7) / 8 * 8 + (size_t)s.chains * 16 + (size_t)s.chains * 4;
     b = (b + 7) / 8 * 8 + 2 * 65 * 8 + (size_t)NT * 8;
     return b;
 }
 
 hipError_t batch_prepare(size_t lds) {
-    // Dynamic LDS above 64 KB must be allowed explicitly for the kernel.
-    return hipFuncSetAttribute((const void*)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    // Dynamic LDS above 64 KB must be allowed explicitly for each kernel.
+    hipError_t e = hipFuncSetAttribute((const void*)k_batch<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_batch<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return e;
 }
 
-hipError_t launch_batch(const BatchArgs& a, int n_samples, int threads, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch, dim3(n_samples), dim3(threads), lds, s, a);
+hipError_t launch_batch(const BatchArgs& a, int n_samples, int chains, size_t lds, hipStream_t s) {
+    if (batch_lanes(chains) == 8)
+        hipLaunchKernelGGL(k_batch<8>, dim3(n_samples), dim3(kBatchThreads), lds, s, a);
+    else
+        hipLaunchKernelGGL(k_batch<4>, dim3(n_samples), dim3(kBatchThreads), lds, s, a);
     return hipGetLastError();
 }
 
