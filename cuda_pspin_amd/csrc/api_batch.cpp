@@ -272,7 +272,7 @@ int pspin_batch_create(const pspin_sim_config* base, int n_samples, const int* N
     b->n_spin = spins.size();
     b->n_perm = perm.size();
     b->n_ex = (size_t)n_samples * 4 * (NT - 1);
-    if (ps::batch_prepare(b->lds) != hipSuccess) {
+    if (ps::batch_prepare(b->chains_per_sample, b->lds) != hipSuccess) {
         ps::set_error("the batch kernel cannot get " + std::to_string(b->lds) + " bytes of LDS");
         return fail(PSPIN_ERR_HIP);
     }

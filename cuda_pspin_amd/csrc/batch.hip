@@ -84,8 +84,8 @@ __device__ __forceinline__ float chain_sum(float v) {
 // the fields of site i over every (L/2)-th entry of its lists, lanes [L/2, L)
 // those of site j, and chain_sum adds the L partial Delta E.  Lane sub = 0
 // writes the accepted amplitudes and owns the chain's counters and energy.
-template <int L>
-__global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
+template <int L, int TB>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256))) void k_batch(BatchArgs a) {
     extern __shared__ double lds[];
     const BatchSample S = a.samples[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -152,35 +152,50 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
                 const BatchDraw d = nd;
                 if (step + 1 < nsteps) nd = dr[(size_t)(step + 1) * C + c];
                 const int i0 = d.ij & 0xffff, j0 = d.ij >> 16;
-                // cube proposal (include/spins_sparse.h:26-56)
+                // this lane's site (i for the first half of the chain's lanes, j for the second);
+                // its list bounds and first entries are read before the proposal's arithmetic
+                const int site = half ? j0 : i0, other = half ? i0 : j0;
+                const int b2 = o2[site] + hs, n2 = o2[site + 1], b4 = o4[site] + hs, n4 = o4[site + 1];
                 const double2 ai = amp[i0 * P + c], aj = amp[j0 * P + c];
+                // an index inside the sample's entry arrays (the value is unused when the list is empty)
+                uint2 en2 = e2[b2 < n2 ? b2 : (n2 > 0 ? n2 - 1 : 0)];
+                uint2 en4 = e4[b4 < n4 ? b4 : (n4 > 0 ? n4 - 1 : 0)];
+                // cube proposal (include/spins_sparse.h:26-56)
                 const double r1 = ai.x * ai.x + ai.y * ai.y, r2 = aj.x * aj.x + aj.y * aj.y;
                 const double sq = sqrt(r1 * r1 + r2 * r2);
                 const double m1 = sqrt(sq * d.q1), m2 = sqrt(sq * d.q2);
                 const double2 ni = make_double2(m1 * d.c1, m1 * d.s1), nj = make_double2(m2 * d.c2, m2 * d.s2);
-                // this lane's site (i for the first half of the chain's lanes, j for the second)
-                const int site = half ? j0 : i0, other = half ? i0 : j0;
                 const double2 as = half ? aj : ai, ns = half ? nj : ni;
                 const double2 ao = half ? ai : aj, no = half ? ni : nj;
                 const float2 dd = make_float2((float)(ns.x - as.x), (float)(ns.y - as.y));
                 const float2 mo = make_float2((float)(0.5 * (ao.x + no.x)), (float)(0.5 * (ao.y + no.y)));
                 // local field at the midpoint configuration, this lane's share of the entries
+                // (every H-th entry in list order; each iteration reads the next entry ahead)
                 float2 F = make_float2(0.f, 0.f);
-                for (int e = o2[site] + hs; e < o2[site + 1]; e += H) {
-                    const uint2 en = e2[e];
+                for (int e = b2; e < n2; e += H) {
+                    const uint2 en = en2;
                     const int k = (int)en.x;
                     const float g = __uint_as_float(en.y);
-                    const float2 t = (k == other) ? mo : f2(amp[k * P + c]);
+                    const double2 Ak = amp[k * P + c];  // unconditional: no branch around the LDS read
+                    en2 = e2[e + H < n2 ? e + H : e];
+                    __builtin_amdgcn_sched_barrier(0);  // both reads in flight together
+                    const float2 tk = f2(Ak);
+                    const float2 t = (k == other) ? mo : tk;
                     F.x += g * t.x;
                     F.y -= g * t.y;
                 }
-                for (int e = o4[site] + hs; e < o4[site + 1]; e += H) {
-                    const uint2 en = e4[e];
+                for (int e = b4; e < n4; e += H) {
+                    const uint2 en = en4;
+                    en4 = e4[e + H < n4 ? e + H : e];
                     const int p = en.x & 0xff, m = (en.x >> 8) & 0xff, n = (en.x >> 16) & 0xff;
                     const float g = __uint_as_float(en.y);
-                    const float2 tp = (p == other) ? mo : f2(amp[p * P + c]);
-                    const float2 tm = (m == other) ? mo : f2(amp[m * P + c]);
-                    const float2 tn = (n == other) ? mo : f2(amp[n * P + c]);
+                    // the three reads go out together; the midpoint replaces the other site's value
+                    const double2 Ap = amp[p * P + c], Am = amp[m * P + c], An = amp[n * P + c];
+                    __builtin_amdgcn_sched_barrier(0);  // keep the three reads in flight together
+                    const float2 ap = f2(Ap), am = f2(Am), an = f2(An);
+                    const float2 tp = (p == other) ? mo : ap;
+                    const float2 tm = (m == other) ? mo : am;
+                    const float2 tn = (n == other) ? mo : an;
                     const float2 mn = make_float2(tm.x * tn.x - tm.y * tn.y, tm.x * tn.y + tm.y * tn.x);
                     F.x += g * (tp.x * mn.x + tp.y * mn.y);
                     F.y += g * (tp.y * mn.x - tp.x * mn.y);
@@ -299,7 +314,26 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
     }
 }
 
+// Lanes per chain and workgroup size: 8 lanes in the smallest of 512, 768 or
+// 1024 threads that holds them (fewer threads leave each more registers), 4
+// lanes in 1024 threads above 128 chains.
 int batch_lanes(int chains) { return chains <= kBatchThreads / 8 ? 8 : 4; }
+
+int batch_threads(int chains) {
+    const int need = chains * batch_lanes(chains);
+    return need <= 512 ? 512 : (need <= 768 ? 768 : 1024);
+}
+
+namespace {
+const void* batch_kernel(int chains) {
+    if (batch_lanes(chains) == 4) return (const void*)k_batch<4, 1024>;
+    switch (batch_threads(chains)) {
+        case 512: return (const void*)k_batch<8, 512>;
+        case 768: return (const void*)k_batch<8, 768>;
+        default: return (const void*)k_batch<8, 1024>;
+    }
+}
+}  // namespace
 
 size_t batch_lds_bytes(const BatchSample& s, int NT) {
     size_t b = (size_t)s.N * (s.chains | 1) * 16 + (size_t)(s.n_e4 + s.n_e2) * 8 + (size_t)2 * (s.N + 1) * 4;
@@ -308,20 +342,14 @@ size_t batch_lds_bytes(const BatchSample& s, int NT) {
     return b;
 }
 
-hipError_t batch_prepare(size_t lds) {
-    // Dynamic LDS above 64 KB must be allowed explicitly for each kernel.
-    hipError_t e = hipFuncSetAttribute((const void*)k_batch<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_batch<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    return e;
+hipError_t batch_prepare(int chains, size_t lds) {
+    // Dynamic LDS above 64 KB must be allowed explicitly for the kernel.
+    return hipFuncSetAttribute(batch_kernel(chains), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 hipError_t launch_batch(const BatchArgs& a, int n_samples, int chains, size_t lds, hipStream_t s) {
-    if (batch_lanes(chains) == 8)
-        hipLaunchKernelGGL(k_batch<8>, dim3(n_samples), dim3(kBatchThreads), lds, s, a);
-    else
-        hipLaunchKernelGGL(k_batch<4>, dim3(n_samples), dim3(kBatchThreads), lds, s, a);
-    return hipGetLastError();
+    void* args[] = {(void*)&a};
+    return hipLaunchKernel(batch_kernel(chains), dim3(n_samples), dim3(batch_threads(chains)), args, lds, s);
 }
 
 }  // namespace ps

@@ -55,11 +55,12 @@ struct BatchArgs {
     int max_steps;      // largest N / 2 of the batch
 };
 
-constexpr int kBatchThreads = 1024;  // workgroup of the batch kernel (draws: all threads; chains: 4 or 8 lanes each)
+constexpr int kBatchThreads = 1024;  // largest workgroup of the batch kernel (draws: all threads; chains: 4 or 8 lanes each)
 
-int batch_lanes(int chains);  // lanes per chain: 8 up to 128 chains per sample, else 4
+int batch_lanes(int chains);    // lanes per chain: 8 up to 128 chains per sample, else 4
+int batch_threads(int chains);  // workgroup size: 512, 768 or 1024
 size_t batch_lds_bytes(const BatchSample& s, int NT);
-hipError_t batch_prepare(size_t lds);
+hipError_t batch_prepare(int chains, size_t lds);
 hipError_t launch_batch(const BatchArgs& a, int n_samples, int chains, size_t lds, hipStream_t s);
 
 }  // namespace ps

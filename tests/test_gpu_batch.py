@@ -1,5 +1,5 @@
-"""Batched sparse PT (pspin_batch_*: one workgroup per disorder sample, one
-lane per chain) against single PSPIN_SPARSE states with the same seeds and
+"""Batched sparse PT (pspin_batch_*: one workgroup per disorder sample, 4 or
+8 lanes per chain) against single PSPIN_SPARSE states with the same seeds and
 against the FP64 oracle.
 
 A batch sample starts from the same disorder, spins and RNG seeds as the
@@ -50,6 +50,33 @@ def test_batch_matches_single_states(pspin, oracle, pt_freq):
         Es, as_, ps_ = st.results()
         np.testing.assert_allclose(b.spins(s), st.spins(), atol=1e-9)
         # running energies accumulate FP32 Delta E in a different order
+        np.testing.assert_allclose(Eb, Es, rtol=0, atol=1e-5 * N)
+        np.testing.assert_array_equal(ab, as_)
+        np.testing.assert_array_equal(pb, ps_)
+        st.close()
+    b.close()
+
+
+@pytest.mark.parametrize("NT,nr", [(4, 24), (4, 32), (5, 32)])
+def test_batch_workgroup_shapes_match_single_states(pspin, oracle, NT, nr):
+    """96, 128 and 160 chains per sample: 8 lanes per chain in 768 and 1024
+    threads, then 4 lanes per chain (more than 128 chains)."""
+    Ns, seeds = [12, 16], [505, 606]
+    T, beta = oracle.temperatures(NT, 0.4, 1.5, 1)
+    base = pspin.SimConfig.default(fmc_mode=2, gamma=0.2)
+    b = pspin.PSpinBatch(base, Ns, seeds, NT, nr, beta)
+    b.set_resync_interval(0)
+    sweeps = 12
+    b.run(sweeps, 1)
+    for s, (N, seed) in enumerate(zip(Ns, seeds)):
+        print(f"[batch shapes] chains={NT * nr} N={N} seed={seed}", flush=True)
+        st = _single(pspin, N, seed, NT, nr, beta, 2, 0.2)
+        st.set_resync_interval(0)
+        st.pt_run(sweeps, 1)
+        np.testing.assert_array_equal(b.perm(s), st.pt_perm())
+        Eb, ab, pb = b.results(s)
+        Es, as_, ps_ = st.results()
+        np.testing.assert_allclose(b.spins(s), st.spins(), atol=1e-9)
         np.testing.assert_allclose(Eb, Es, rtol=0, atol=1e-5 * N)
         np.testing.assert_array_equal(ab, as_)
         np.testing.assert_array_equal(pb, ps_)
