@@ -6,22 +6,25 @@
 // Each sparse step touches only ~4N/N = 4 quartet terms per site plus the
 // few FMC-surviving g2 neighbours, so a workgroup per replica (as in
 // k_sweep) spends its time in barriers.  Here one workgroup owns one disorder
-// sample and one LANE owns one chain (replica x temperature) of it: a chain
-// steps through its sweep with no barriers, reading its own FP64 amplitudes
-// from LDS and the sample's per-site term lists from LDS.  After every
-// pt_freq sweeps the workgroup meets once, wave 0 runs the replica exchange
-// with the reference's decision order and drand48 stream for that sample,
-// and every chain picks up its new beta.
+// sample and a group of 8 (or 4) consecutive lanes owns one chain (replica x
+// temperature) of it: a chain steps through its sweep with no barriers,
+// reading its FP64 amplitudes and the sample's per-site term lists from LDS.
+// The first half of the chain's lanes sums the local field of site i, the
+// second half that of site j, each lane every (L/2)-th entry of the lists,
+// and a DPP sum combines them.  After every pt_freq sweeps the workgroup
+// meets once, wave 0 runs the replica exchange with the reference's decision
+// order and drand48 stream for that sample, and every chain picks up its new
+// beta.
 //
 // A sweep runs in two phases.  First every thread of the workgroup makes the
 // state-independent draws of all (step, chain) pairs of the sample (Philox
-// blocks, pair, phases, intensity split, -log u of the threshold) into a
-// global scratch array, [step][chain]; these FP64 transcendentals are most of
-// a step's instructions, and this way they run with the whole workgroup's
-// waves hiding their latency.  Then each chain's lane walks its steps,
-// prefetching the next step's draws, with only the amplitude-dependent part
-// left: three square roots of the cube proposal, the local fields and the
-// decision.
+// blocks, pair, phases, intensity split, the Metropolis threshold with the
+// chain's beta for the sweep) into a global scratch array, [step][chain];
+// these FP64 transcendentals are most of a step's instructions, and this way
+// they run with the whole workgroup's waves hiding their latency.  Then each
+// chain's lanes walk its steps, prefetching the next step's draws, with only
+// the amplitude-dependent part left: three square roots of the cube proposal,
+// the local fields and the decision.
 //
 // Per chain the arithmetic is the dense kernel's: FP64 cube proposal
 // (include/spins_sparse.h:26-56), FP32 midpoint local-field Delta E
@@ -40,7 +43,9 @@ __device__ __forceinline__ float2 f2(double2 v) { return make_float2((float)v.x,
 // The draws of one step of one chain (k_sweep's make_step with the cube
 // proposal's amplitude-independent factors; src/mc_sparse.cu:118-126,
 // include/spins_sparse.h:34-41).
-__device__ __forceinline__ BatchDraw batch_draw(uint64_t seed, uint64_t sweep, int step, int N) {
+// The threshold -log(u) / beta uses the chain's beta for the sweep (beta = 0:
+// every proposal is accepted).
+__device__ __forceinline__ BatchDraw batch_draw(uint64_t seed, uint64_t sweep, int step, int N, double beta) {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep0}, k0, k1);
     const U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep1}, k0, k1);
@@ -57,7 +62,7 @@ __device__ __forceinline__ BatchDraw batch_draw(uint64_t seed, uint64_t sweep, i
     sincos(2.0 * M_PI * u32_to_unit(y.x), &d.s2, &d.c2);
     d.q1 = sqrt(1.0 - u);
     d.q2 = sqrt(u);
-    d.lu = -log(u32_to_unit(y.y));
+    d.thr = (beta == 0.0) ? INFINITY : -log(u32_to_unit(y.y)) / beta;
     return d;
 }
 
@@ -117,7 +122,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
         const double* g = a.spins + S.spin_off + (size_t)x * 2;
         amp[site * P + cc] = make_double2(g[0], g[1]);
     }
-    if (tid < C) sperm[tid] = a.perm[S.chain_off + tid];
+    if (tid < C) {
+        sperm[tid] = a.perm[S.chain_off + tid];
+        sB[tid] = a.betas[S.chain_off + tid];  // kept current by the exchange; the draws read it
+    }
     if (tid < 64) lcg48_coeffs((uint64_t)tid + 1, &jA[tid + 1], &jC[tid + 1]);
     if (tid == 0) {
         jA[0] = 1;
@@ -142,7 +150,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
         for (int x = tid; x < nsteps * C; x += blockDim.x) {
             const int st = x / C, cx = x - st * C;
             const int gc = S.chain_off + cx;
-            dr[x] = batch_draw(a.seeds[gc], a.ctr[gc] + (uint64_t)sw, st, N);
+            dr[x] = batch_draw(a.seeds[gc], a.ctr[gc] + (uint64_t)sw, st, N, sB[cx]);
         }
         __syncthreads();
         // Phase 2: each chain's steps, the next step's draws loaded one ahead.
@@ -201,9 +209,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
                     F.y += g * (tp.y * mn.x - tp.x * mn.y);
                 }
                 const float dE = chain_sum<L>(-(dd.x * F.x - dd.y * F.y));
-                const double thr = (beta == 0.0) ? INFINITY : d.lu / beta;
                 nprop++;
-                if (beta == 0.0 || (double)dE < thr) {
+                if (beta == 0.0 || (double)dE < d.thr) {
                     if (sub == 0) {
                         amp[i0 * P + c] = ni;
                         amp[j0 * P + c] = nj;
