@@ -108,7 +108,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
     int* sperm = (int*)(sB + C);                        // [C]
     uint64_t* jA = (uint64_t*)(((uintptr_t)(sperm + C) + 7) & ~(uintptr_t)7);  // [65]
     uint64_t* jC = jA + 65;
-    long long* sacc = (long long*)(jC + 65);            // [NT]
+    long long* sacc = (long long*)(jC + 65);            // [NT] accepted swaps per level over the launch
+    double* sBs = (double*)(sacc + NT);                 // [NT] beta schedule
+    double* sDb = sBs + NT;                             // [NT] beta_t - beta_{t+1}
 
     // ---- stage the sample's terms and the chains' amplitudes
     for (int i = tid; i < S.n_e4; i += blockDim.x) e4[i] = a.e4[S.e4_off + i];
@@ -131,11 +133,16 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
         jA[0] = 1;
         jC[0] = 0;
     }
+    if (tid < NT) {
+        sBs[tid] = a.beta_sched[tid];
+        sacc[tid] = 0;
+        if (tid < NT - 1) sDb[tid] = a.beta_sched[tid] - a.beta_sched[tid + 1];
+    }
     const int ch = S.chain_off + c;
     double beta = live ? a.betas[ch] : 0.0;
     double E = live ? a.E[ch] : 0.0;
     uint64_t sweep = live ? a.ctr[ch] : 0;
-    long long nacc = 0, nprop = 0;
+    long long nacc = 0, nprop = 0, nex = 0;
     int since = live ? a.since[ch] : 0;
     uint64_t X = (tid < 64) ? a.rng48[blockIdx.x] : 0;  // wave 0 keeps the sample's drand48 state
     const long long done0 = a.done[blockIdx.x];
@@ -257,14 +264,24 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
             __syncthreads();
             if (tid < 64) {
                 const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-                for (int t = 0; t < NT - 1; t++) {
-                    const double db = a.beta_sched[t] - a.beta_sched[t + 1];
-                    long long na = 0;
-                    for (int c0 = 0; c0 < nr; c0 += 64) {
-                        const int r = c0 + lane;
-                        const bool valid = r < nr;
-                        double delta = 0.0;
-                        if (valid) delta = db * (sE[sperm[t * nr + r]] - sE[sperm[(t + 1) * nr + r]]);
+                if (nr <= 64) {
+                    // Lane r carries the chain that climbs column r: at level t it
+                    // holds the chain that levels < t left at (t, r).  The chain at
+                    // (t + 1, r) is not written before level t, so it and its
+                    // energy are read a level ahead, off the serial path.
+                    const bool valid = lane < nr;
+                    int cur = valid ? sperm[lane] : 0;
+                    double Ecur = valid ? sE[cur] : 0.0;
+                    int nx = valid ? sperm[nr + lane] : 0;
+                    double Enx = valid ? sE[nx] : 0.0;
+                    for (int t = 0; t < NT - 1; t++) {
+                        const int n1 = nx;
+                        const double E1 = Enx;
+                        if (valid && t + 2 < NT) {
+                            nx = sperm[(t + 2) * nr + lane];
+                            Enx = sE[nx];
+                        }
+                        const double delta = sDb[t] * (Ecur - E1);
                         const bool need = valid && !(delta >= 0.0);
                         const unsigned long long m = __ballot(need);
                         bool accept = valid;
@@ -274,26 +291,60 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
                         }
                         const int tot = __popcll(m);
                         X = (jA[tot] * X + jC[tot]) & kMask48;
-                        if (valid && accept) {
-                            const int tmp = sperm[t * nr + r];
-                            sperm[t * nr + r] = sperm[(t + 1) * nr + r];
-                            sperm[(t + 1) * nr + r] = tmp;
+                        if (valid) {
+                            sperm[t * nr + lane] = accept ? n1 : cur;
+                            if (!accept) {
+                                cur = n1;
+                                Ecur = E1;
+                            }
                         }
-                        na += __popcll(__ballot(valid && accept));
+                        const unsigned long long am = __ballot(valid && accept);
+                        if (lane == 0) sacc[t] += __popcll(am);
                     }
-                    if (lane == 0) sacc[t] = na;
+                    if (valid) sperm[(NT - 1) * nr + lane] = cur;
+                } else {
+                    for (int t = 0; t < NT - 1; t++) {
+                        long long na = 0;
+                        for (int c0 = 0; c0 < nr; c0 += 64) {
+                            const int r = c0 + lane;
+                            const bool valid = r < nr;
+                            double delta = 0.0;
+                            if (valid) delta = sDb[t] * (sE[sperm[t * nr + r]] - sE[sperm[(t + 1) * nr + r]]);
+                            const bool need = valid && !(delta >= 0.0);
+                            const unsigned long long m = __ballot(need);
+                            bool accept = valid;
+                            if (need) {
+                                const int k = __popcll(m & lt) + 1;
+                                accept = lcg48_unit((jA[k] * X + jC[k]) & kMask48) < exp(delta);
+                            }
+                            const int tot = __popcll(m);
+                            X = (jA[tot] * X + jC[tot]) & kMask48;
+                            if (valid && accept) {
+                                const int tmp = sperm[t * nr + r];
+                                sperm[t * nr + r] = sperm[(t + 1) * nr + r];
+                                sperm[(t + 1) * nr + r] = tmp;
+                            }
+                            na += __popcll(__ballot(valid && accept));
+                        }
+                        if (lane == 0) sacc[t] += na;
+                    }
                 }
-                for (int t = lane; t < NT - 1; t += 64) {
-                    long long* ex = a.ex + (size_t)blockIdx.x * 4 * (NT - 1);
-                    ex[t] += sacc[t];
-                    ex[(NT - 1) + t] += nr;
-                    ex[2 * (NT - 1) + t] += sacc[t];
-                    ex[3 * (NT - 1) + t] += nr;
-                }
-                for (int i = lane; i < C; i += 64) sB[sperm[i]] = a.beta_sched[i / nr];
+                for (int i = lane; i < C; i += 64) sB[sperm[i]] = sBs[i / nr];
             }
+            nex++;
             __syncthreads();
             if (live) beta = sB[c];
+        }
+    }
+
+    // ---- exchange counters of the launch, once
+    if (tid < 64 && nex > 0) {
+        long long* ex = a.ex + (size_t)blockIdx.x * 4 * (NT - 1);
+        for (int t = lane; t < NT - 1; t += 64) {
+            ex[t] += sacc[t];
+            ex[(NT - 1) + t] += nr * nex;
+            ex[2 * (NT - 1) + t] += sacc[t];
+            ex[3 * (NT - 1) + t] += nr * nex;
         }
     }
 
@@ -345,7 +396,7 @@ const void* batch_kernel(int chains) {
 size_t batch_lds_bytes(const BatchSample& s, int NT) {
     size_t b = (size_t)s.N * (s.chains | 1) * 16 + (size_t)(s.n_e4 + s.n_e2) * 8 + (size_t)2 * (s.N + 1) * 4;
     b = (b + 7) / 8 * 8 + (size_t)s.chains * 16 + (size_t)s.chains * 4;
-    b = (b + 7) / 8 * 8 + 2 * 65 * 8 + (size_t)NT * 8;
+    b = (b + 7) / 8 * 8 + 2 * 65 * 8 + (size_t)NT * 8 * 3;
     return b;
 }
 

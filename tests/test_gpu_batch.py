@@ -57,10 +57,11 @@ def test_batch_matches_single_states(pspin, oracle, pt_freq):
     b.close()
 
 
-@pytest.mark.parametrize("NT,nr", [(4, 24), (4, 32), (5, 32)])
+@pytest.mark.parametrize("NT,nr", [(4, 24), (4, 32), (5, 32), (2, 96)])
 def test_batch_workgroup_shapes_match_single_states(pspin, oracle, NT, nr):
-    """96, 128 and 160 chains per sample: 8 lanes per chain in 768 and 1024
-    threads, then 4 lanes per chain (more than 128 chains)."""
+    """96, 128, 160 and 192 chains per sample: 8 lanes per chain in 768 and
+    1024 threads, then 4 lanes per chain (more than 128 chains); 96 replicas
+    per temperature take the exchange's path for more than 64 per level."""
     Ns, seeds = [12, 16], [505, 606]
     T, beta = oracle.temperatures(NT, 0.4, 1.5, 1)
     base = pspin.SimConfig.default(fmc_mode=2, gamma=0.2)
@@ -74,6 +75,10 @@ def test_batch_workgroup_shapes_match_single_states(pspin, oracle, NT, nr):
         st.set_resync_interval(0)
         st.pt_run(sweeps, 1)
         np.testing.assert_array_equal(b.perm(s), st.pt_perm())
+        _, _, acc_s, prop_s = st.pt_exchange_stats()
+        acc_b, prop_b = b.exchange_stats(s)
+        np.testing.assert_array_equal(acc_b, acc_s)
+        np.testing.assert_array_equal(prop_b, prop_s)
         Eb, ab, pb = b.results(s)
         Es, as_, ps_ = st.results()
         np.testing.assert_allclose(b.spins(s), st.spins(), atol=1e-9)
