@@ -16,15 +16,17 @@
 // order and drand48 stream for that sample, and every chain picks up its new
 // beta.
 //
-// A sweep runs in two phases.  First every thread of the workgroup makes the
-// state-independent draws of all (step, chain) pairs of the sample (Philox
-// blocks, pair, phases, intensity split, the Metropolis threshold with the
-// chain's beta for the sweep) into a global scratch array, [step][chain];
-// these FP64 transcendentals are most of a step's instructions, and this way
-// they run with the whole workgroup's waves hiding their latency.  Then each
-// chain's lanes walk its steps, prefetching the next step's draws, with only
-// the amplitude-dependent part left: three square roots of the cube proposal,
-// the local fields and the decision.
+// A sweep runs in two phases.  First the workgroup makes the state-independent
+// draws of all (step, chain) pairs of the sample (Philox blocks, pair, phases,
+// intensity split, -log u of the threshold) into a global scratch array,
+// [step][chain]; these FP64 transcendentals are most of a step's
+// instructions, and this way they run with the whole workgroup's waves hiding
+// their latency.  Then each chain's lanes walk its steps, prefetching the next
+// step's draws, with only the amplitude-dependent part left: three square
+// roots of the cube proposal, the local fields and the decision.  The draws
+// do not depend on beta, so after a sweep that ends in an exchange, wave 0
+// runs the (serial) exchange while the other waves make the next sweep's
+// draws.
 //
 // Per chain the arithmetic is the dense kernel's: FP64 cube proposal
 // (include/spins_sparse.h:26-56), FP32 midpoint local-field Delta E
@@ -43,9 +45,7 @@ __device__ __forceinline__ float2 f2(double2 v) { return make_float2((float)v.x,
 // The draws of one step of one chain (k_sweep's make_step with the cube
 // proposal's amplitude-independent factors; src/mc_sparse.cu:118-126,
 // include/spins_sparse.h:34-41).
-// The threshold -log(u) / beta uses the chain's beta for the sweep (beta = 0:
-// every proposal is accepted).
-__device__ __forceinline__ BatchDraw batch_draw(uint64_t seed, uint64_t sweep, int step, int N, double beta) {
+__device__ __forceinline__ BatchDraw batch_draw(uint64_t seed, uint64_t sweep, int step, int N) {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep0}, k0, k1);
     const U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep1}, k0, k1);
@@ -62,7 +62,7 @@ __device__ __forceinline__ BatchDraw batch_draw(uint64_t seed, uint64_t sweep, i
     sincos(2.0 * M_PI * u32_to_unit(y.x), &d.s2, &d.c2);
     d.q1 = sqrt(1.0 - u);
     d.q2 = sqrt(u);
-    d.thr = (beta == 0.0) ? INFINITY : -log(u32_to_unit(y.y)) / beta;
+    d.lu = -log(u32_to_unit(y.y));
     return d;
 }
 
@@ -124,10 +124,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
         const double* g = a.spins + S.spin_off + (size_t)x * 2;
         amp[site * P + cc] = make_double2(g[0], g[1]);
     }
-    if (tid < C) {
-        sperm[tid] = a.perm[S.chain_off + tid];
-        sB[tid] = a.betas[S.chain_off + tid];  // kept current by the exchange; the draws read it
-    }
+    if (tid < C) sperm[tid] = a.perm[S.chain_off + tid];
     if (tid < 64) lcg48_coeffs((uint64_t)tid + 1, &jA[tid + 1], &jC[tid + 1]);
     if (tid == 0) {
         jA[0] = 1;
@@ -150,16 +147,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
 
     const int nsteps = N / 2;
     BatchDraw* dr = a.draws + (size_t)S.chain_off * a.max_steps;  // this sample's [step][chain] draws
-    for (int sw = 0; sw < a.n_sweeps; sw++) {
-        // Phase 1: every (step, chain) draw of the sweep, all threads (after the
-        // chains have read the previous sweep's draws).
-        if (sw > 0) __syncthreads();
-        for (int x = tid; x < nsteps * C; x += blockDim.x) {
+    // Phase 1 of sweep sw: every (step, chain) draw, by threads [first, blockDim.x).
+    auto make_draws = [&](int sw, int first) {
+        for (int x = tid - first; x < nsteps * C; x += (int)blockDim.x - first) {
             const int st = x / C, cx = x - st * C;
             const int gc = S.chain_off + cx;
-            dr[x] = batch_draw(a.seeds[gc], a.ctr[gc] + (uint64_t)sw, st, N, sB[cx]);
+            dr[x] = batch_draw(a.seeds[gc], a.ctr[gc] + (uint64_t)sw, st, N);
         }
-        __syncthreads();
+    };
+    if (a.n_sweeps > 0) make_draws(0, 0);
+    __syncthreads();
+    for (int sw = 0; sw < a.n_sweeps; sw++) {
         // Phase 2: each chain's steps, the next step's draws loaded one ahead.
         if (live) {
             BatchDraw nd = dr[c];
@@ -182,6 +180,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
                 const double2 ni = make_double2(m1 * d.c1, m1 * d.s1), nj = make_double2(m2 * d.c2, m2 * d.s2);
                 const double2 as = half ? aj : ai, ns = half ? nj : ni;
                 const double2 ao = half ? ai : aj, no = half ? ni : nj;
+                const double thr = (beta == 0.0) ? INFINITY : d.lu / beta;
                 const float2 dd = make_float2((float)(ns.x - as.x), (float)(ns.y - as.y));
                 const float2 mo = make_float2((float)(0.5 * (ao.x + no.x)), (float)(0.5 * (ao.y + no.y)));
                 // local field at the midpoint configuration, this lane's share of the entries
@@ -217,7 +216,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
                 }
                 const float dE = chain_sum<L>(-(dd.x * F.x - dd.y * F.y));
                 nprop++;
-                if (beta == 0.0 || (double)dE < d.thr) {
+                if (beta == 0.0 || (double)dE < thr) {
                     if (sub == 0) {
                         amp[i0 * P + c] = ni;
                         amp[j0 * P + c] = nj;
@@ -258,10 +257,12 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
             }
         }
         const bool exch = a.pt_freq > 0 && (done0 + sw + 1) % a.pt_freq == 0;
+        if (exch && lead) sE[c] = E;
+        __syncthreads();  // the chains are done with this sweep's draws and have published their energies
+        // Wave 0 runs the exchange while the other waves make the next sweep's draws.
+        if (sw + 1 < a.n_sweeps && (!exch || tid >= 64)) make_draws(sw + 1, exch ? 64 : 0);
         if (exch) {
             // Exchange (src/parallel_tempering.cu:316-352) for this sample.
-            if (lead) sE[c] = E;
-            __syncthreads();
             if (tid < 64) {
                 const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
                 if (nr <= 64) {
@@ -332,9 +333,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
                 for (int i = lane; i < C; i += 64) sB[sperm[i]] = sBs[i / nr];
             }
             nex++;
-            __syncthreads();
-            if (live) beta = sB[c];
         }
+        __syncthreads();  // the next sweep's draws are made and the new betas published
+        if (exch && live) beta = sB[c];
     }
 
     // ---- exchange counters of the launch, once

@@ -24,7 +24,7 @@ struct BatchDraw {
     int pad;
     double c1, s1, c2, s2;  // cos and sin of the two phases
     double q1, q2;          // sqrt(1 - u), sqrt(u) of the intensity split u
-    double thr;             // -log(u_accept) / beta (infinite at beta = 0)
+    double lu;              // -log(u_accept); the threshold is lu / beta
 };
 
 struct BatchArgs {
