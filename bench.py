@@ -246,6 +246,17 @@ def load_traffic_key(key):
     return rec.get(key, {}).get("hbm_bytes_per_launch")
 
 
+def batch_traffic(sweeps_per_launch):
+    """HBM bytes of one batch launch of sweeps_per_launch sweeps, from the
+    measured bytes per sweep (the per-sweep draw scratch dominates)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        per = json.load(open(path)).get("batch_256", {}).get("hbm_bytes_per_sweep")
+    except (OSError, ValueError):
+        return None
+    return None if per is None else per * sweeps_per_launch
+
+
 def splitmix64(state):
     z = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
     z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
@@ -289,7 +300,7 @@ def run_batch(args):
                       "autotuned_sweeps_per_launch": chunk, "autotune_trial_ms_per_sweep": trial_ms},
            "init_s": t_init,
            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK, "traffic": load_traffic_key("batch_256"),
+                        "frac": achieved / HBM_PEAK, "traffic": batch_traffic(chunk),
                         "note": "terms and amplitudes are LDS-resident; latency-bound per lane"}}
     b.close()
     print(json.dumps(out))

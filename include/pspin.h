@@ -178,8 +178,9 @@ int pspin_pt_get_series(const pspin_state* st, double* out, int* n_out);
  * over labels).  A batch holds n_samples independent PSPIN_SPARSE systems,
  * sample s with N = Ns[s] and seed = seeds[s] (other fields from `base`), each
  * with NT x nrep_per_temp chains on the geometric/linear schedule beta_sched,
- * and runs them all in one launch: one workgroup per sample, one lane per
- * chain, the exchange of src/parallel_tempering.cu:316-352 per sample with
+ * and runs them all in one launch: one workgroup per sample, 8 lanes per
+ * chain (4 above 128 chains per sample; NT * nrep_per_temp <= 256), the
+ * exchange of src/parallel_tempering.cu:316-352 per sample with
  * srand48(seed + 5000).  Sample s starts from the same disorder, spins and
  * RNG seeds as pspin_create(.., PSPIN_SPARSE) with that seed. */
 typedef struct pspin_batch pspin_batch;
