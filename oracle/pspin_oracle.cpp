@@ -687,7 +687,7 @@ static double lcg48_next(uint64_t* x) {
 void or_cpu_pt(int N, int NT, int nr, double* spins, const double* g2, int h2_active, const double* g4,
                const uint8_t* mask, const double* beta_sched, const uint64_t* rseeds, int n_sweeps, long seed48,
                int measure_from, double* e_sum, double* e2_sum, double* q2_sum, long long* acc, long long* prop,
-               long long* ex_acc, long long* ex_prop, int* n_meas) {
+               long long* ex_acc, long long* ex_prop, int* n_meas, int qbins, double* qh_sum) {
     const int R = NT * nr;
     std::vector<int> perm(R), slot(R);
     for (int i = 0; i < R; i++) perm[i] = i;
@@ -739,7 +739,12 @@ void or_cpu_pt(int N, int NT, int nr, double* spins, const double* g2, int h2_ac
                         qr += a0[2 * i] * a1[2 * i] + a0[2 * i + 1] * a1[2 * i + 1];
                         qi += a0[2 * i + 1] * a1[2 * i] - a0[2 * i] * a1[2 * i + 1];
                     }
-                    q2_sum[t] += (qr * qr + qi * qi) / ((double)N * N);
+                    const double q2 = (qr * qr + qi * qi) / ((double)N * N);
+                    q2_sum[t] += q2;
+                    if (qbins > 0) {  // histogram of |q| over [0, 1] (|q| <= 1 on the sphere)
+                        const int b = std::min(qbins - 1, (int)(std::sqrt(q2) * qbins));
+                        qh_sum[(size_t)t * qbins + b] += 1.0;
+                    }
                 }
             }
             (*n_meas)++;

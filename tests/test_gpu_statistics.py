@@ -7,10 +7,12 @@ run on the same disorder from the same initial spins with independent RNG
 streams.  Per sample the difference of each observable is formed; over the
 samples the mean difference and its jackknife error give z-scores (the
 reference's analysis averages over samples with jackknife,
-src/analysis_pt.cpp:846-900).  Observables: <E/N> and <|q|^2> per temperature
-(q = sum_k a_k^1 conj(a_k^2) / N between the two replicas of a slot, the
-overlap of notes/quantities/observables.tex), the Metropolis acceptance and
-the exchange rate per temperature pair.
+src/analysis_pt.cpp:846-900).  Observables per temperature: <E/N>, the
+specific heat Cv = N beta^2 (<e^2> - <e>^2) (src/analysis_pt.cpp:846-900),
+<|q|^2> and the histogram P(|q|) in QB bins over [0, 1] (q = sum_k a_k^1
+conj(a_k^2) / N between the two replicas of a slot, the overlap of
+notes/quantities/observables.tex); the Metropolis acceptance and the
+exchange rate per temperature pair.
 """
 import concurrent.futures as cf
 
@@ -21,9 +23,11 @@ pytestmark = pytest.mark.gpu
 
 N, NT, NR = 10, 5, 2
 BURN, MEAS = 500, 2000
+QB = 4  # |q| histogram bins
 
 
 def _gpu_run(pspin, seed, beta, groups):
+    beta = np.asarray(beta)
     cfg = pspin.SimConfig.default(N=N, nrep=NT * NR, seed=seed, fmc_mode=0)
     st = pspin.PSpin(cfg, pspin.DENSE)
     st.set_pair_schedule(groups)
@@ -36,7 +40,9 @@ def _gpu_run(pspin, seed, beta, groups):
     st.reset_counters()
     st.pt_reset_window()
     e = np.zeros(NT)
+    e2 = np.zeros(NT)
     q2 = np.zeros(NT)
+    qh = np.zeros((NT, QB))
     n = 0
     for _ in range(MEAS // 2):
         st.pt_run(2, 1)
@@ -44,25 +50,30 @@ def _gpu_run(pspin, seed, beta, groups):
         perm = st.pt_perm().reshape(NT, NR)
         z = st.spins()
         e += (E[perm] / N).mean(axis=1)
+        e2 += ((E[perm] / N) ** 2).mean(axis=1)
         q = np.sum(z[perm[:, 0]] * np.conj(z[perm[:, 1]]), axis=1) / N
         q2 += np.abs(q) ** 2
+        qh[np.arange(NT), np.minimum(QB - 1, (np.abs(q) * QB).astype(int))] += 1
         n += 1
     _, a, p = st.results()
     xa, xp, _, _ = st.pt_exchange_stats()
     st.close()
-    obs = np.concatenate([e / n, q2 / n, [a.sum() / p.sum()], xa / xp])
+    cv = N * beta ** 2 * (e2 / n - (e / n) ** 2)
+    obs = np.concatenate([e / n, cv, q2 / n, (qh / n).reshape(-1), [a.sum() / p.sum()], xa / xp])
     return obs, dict(s0=s0, g2=g2, g4=g4, mask=mask, rs=rs)
 
 
 def _cpu_run(oracle, seed, beta, d):
+    beta = np.asarray(beta)
     rs = d["rs"] ^ np.uint64(0x9E3779B97F4A7C15)
     out = oracle.cpu_pt(N, NT, NR, d["s0"], d["g2"], 1, d["g4"], d["mask"], beta, rs, BURN + MEAS,
-                        seed + 5000 + 17, BURN)
+                        seed + 5000 + 17, BURN, qbins=QB)
     n = out["n"]
     e = out["e"] / (n * NR)
+    cv = N * beta ** 2 * (out["e2"] / (n * NR) - e ** 2)
     q2 = out["q2"] / n
     acc = out["acc"].sum() / out["prop"].sum()
-    return np.concatenate([e, q2, [acc], out["ex_acc"] / out["ex_prop"]])
+    return np.concatenate([e, cv, q2, (out["qh"] / n).reshape(-1), [acc], out["ex_acc"] / out["ex_prop"]])
 
 
 @pytest.mark.parametrize("groups", [0, NR])
@@ -84,8 +95,8 @@ def test_gpu_pt_matches_reference_logic_statistically(pspin, oracle, groups):
     jk = np.array([np.delete(diff, i, axis=0).mean(axis=0) for i in range(S)])
     se = np.sqrt((S - 1) / S * np.sum((jk - jk.mean(axis=0)) ** 2, axis=0))
     z = diff.mean(axis=0) / np.maximum(se, 1e-12)
-    names = ([f"e[T={t:.3f}]" for t in T] + [f"q2[T={t:.3f}]" for t in T] + ["acc"] +
-             [f"ex[{t}]" for t in range(NT - 1)])
+    names = ([f"e[T={t:.3f}]" for t in T] + [f"Cv[T={t:.3f}]" for t in T] + [f"q2[T={t:.3f}]" for t in T] +
+             [f"P|q|{b}[T={t:.3f}]" for t in T for b in range(QB)] + ["acc"] + [f"ex[{t}]" for t in range(NT - 1)])
     for nm, g, c, zz in zip(names, gpu.mean(axis=0), cpu.mean(axis=0), z):
         print(f"  {nm:14s} gpu {g: .5f}  cpu {c: .5f}  z {zz: .2f}")
     assert np.all(np.abs(z) < 3.5), z

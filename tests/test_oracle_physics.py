@@ -264,3 +264,27 @@ def test_sparse_select_structure(oracle):
         qq = oracle.encode(i, j, k, l)
         assert mask[qq] == (1 << ch)
         assert abs(gv - g_unit[qq] * math.sqrt(N / len(q))) < 1e-15
+
+
+def test_cpu_pt_overlap_histogram_brackets_q2(oracle):
+    """The |q| histogram of the CPU PT loop counts every measurement once per
+    temperature, and its bin edges bracket the accumulated |q|^2: binning q^2,
+    or an off-by-one bin, breaks one of the two bounds."""
+    N, NT, nr, Q = 6, 3, 2, 8
+    rng = np.random.default_rng(12)
+    g2 = rng.normal(size=(N, N))
+    g2 = (g2 + g2.T) / 2
+    g4 = rng.normal(size=oracle.n_quartets(N))
+    mask = np.ones(oracle.n_quartets(N), dtype=np.uint8)
+    a0 = rng.normal(size=(NT * nr, 2 * N))
+    a0 *= math.sqrt(N) / np.linalg.norm(a0, axis=1, keepdims=True)
+    beta = np.array([0.2, 0.6, 1.8])
+    out = oracle.cpu_pt(N, NT, nr, a0.reshape(-1), g2, 1, g4, mask, beta, oracle.replica_seeds(31, NT * nr),
+                        3000, 5, 500, qbins=Q)
+    n, qh, q2 = out["n"], out["qh"], out["q2"]
+    assert n == 2500
+    np.testing.assert_array_equal(qh.sum(axis=1), np.full(NT, float(n)))
+    lo = (np.arange(Q) / Q) ** 2
+    hi = (np.arange(1, Q + 1) / Q) ** 2
+    assert np.all(qh @ lo <= q2 + 1e-9) and np.all(q2 <= qh @ hi + 1e-9), (qh @ lo, q2, qh @ hi)
+    assert np.all(qh[:, 1:].sum(axis=1) > 0)  # the histogram is not degenerate
