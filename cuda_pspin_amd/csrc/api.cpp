@@ -69,6 +69,7 @@ void free_all(pspin_state* st) {
     dfree(st->d_grp_off); dfree(st->d_grp_hdr); dfree(st->d_grp_g);
     dfree(st->d_spins); dfree(st->d_E); dfree(st->d_betas); dfree(st->d_acc); dfree(st->d_prop);
     dfree(st->d_seeds); dfree(st->d_e2); dfree(st->d_e4); dfree(st->d_ctr);
+    dfree(st->d_echunk); dfree(st->d_epart);
     if (st->graph) (void)hipGraphExecDestroy(st->graph);
     st->graph = nullptr;
     dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48); dfree(st->d_done);
@@ -826,6 +827,13 @@ int pspin_create(const pspin_sim_config* cfg, int variant, pspin_state** out) {
     PS_CREATE_HIP(dmalloc(&st->d_E, nrep));
     PS_CREATE_HIP(dmalloc(&st->d_e2, nrep));
     PS_CREATE_HIP(dmalloc(&st->d_e4, nrep));
+    if (variant == PSPIN_DENSE) {
+        st->n_echunk = ps::energy_chunk_count(N, nrep);
+        const std::vector<int> ch = ps::energy_chunks(N, st->n_echunk);
+        PS_CREATE_HIP(dmalloc(&st->d_echunk, ch.size()));
+        PS_CREATE_HIP(cpy(st, st->d_echunk, ch.data(), ch.size() * sizeof(int), hipMemcpyHostToDevice));
+        PS_CREATE_HIP(dmalloc(&st->d_epart, (size_t)st->n_echunk * nrep));
+    }
     PS_CREATE_HIP(dmalloc(&st->d_betas, nrep));
     PS_CREATE_HIP(dmalloc(&st->d_acc, nrep));
     PS_CREATE_HIP(dmalloc(&st->d_prop, nrep));

@@ -7,11 +7,22 @@
 // fixed order, so results are deterministic for identical inputs (the
 // reference's atomicAdd is not).
 //
-// Dense layout: one block per replica; thread t walks the (k,l) pairs
-// t, t+B, ... and, for each, the contiguous colex run q = C(l,4)+C(k,3)+C(j,2)+i
-// over j<k, i<j -- so the coupling and mask reads are unit-stride and the
-// quartet indices need no decode.  Spins of the replica sit in LDS.
+// Dense layout: the quartets of a (k,l) pair form the contiguous colex run
+// q = C(l,4) + C(k,3) + r, r = C(j,2) + i over i < j < k.  A workgroup takes
+// kEnRep replicas (amplitudes in LDS) and a chunk of pairs (the chunks hold
+// equal quartet counts); each wave takes every 4th pair of the chunk and its
+// lanes take every 64th quartet of the run, so the coupling and mask reads
+// are coalesced and each is used for kEnRep replicas.  The products of a_k
+// and a_l are formed once per run and replica:
+//   ch0  a_i a_j a_k* a_l* = a_i [a_j (a_k a_l)*]
+//   ch1  a_i a_j* a_k* a_l = a_i [a_j* (a_k* a_l)]
+//   ch2  a_i a_j* a_k a_l* = a_i [a_j* (a_k a_l*)]
+// Each workgroup writes one partial sum per replica; a second kernel adds the
+// H2 part and the partials in chunk order.
 #include "pspin_internal.h"
+
+#include <algorithm>
+#include <vector>
 
 namespace ps {
 
@@ -58,45 +69,92 @@ __device__ __forceinline__ double quartet_re(c64 a0, c64 a1, c64 a2, c64 a3, int
 }
 
 constexpr int kEnergyBS = 256;
+constexpr int kEnRep = 4;  // replicas per workgroup of the dense H4 kernel
 
-__global__ __launch_bounds__(kEnergyBS) void k_energy_dense(const double* __restrict__ spins,
-                                                            const double* __restrict__ g2, int h2_active,
-                                                            const double* __restrict__ g4,
-                                                            const uint8_t* __restrict__ mask, int N,
-                                                            double* __restrict__ e2, double* __restrict__ e4) {
+// H4 partial sums of kEnRep replicas over one chunk of (k,l) pairs.
+__global__ __launch_bounds__(kEnergyBS) void k_energy_dense_part(const double* __restrict__ spins, int nrep,
+                                                                 const double* __restrict__ g4,
+                                                                 const uint8_t* __restrict__ mask, int N,
+                                                                 const int* __restrict__ chunks,
+                                                                 double* __restrict__ part) {
+    __shared__ c64 s[kEnRep][kMaxN];
+    __shared__ double red[kEnergyBS / 64][kEnRep];
+    const int r0 = blockIdx.y * kEnRep, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int e = threadIdx.x; e < kEnRep * N; e += blockDim.x) {
+        const int q = e / N, k = e - q * N;
+        c64 v = {0.0, 0.0};
+        if (r0 + q < nrep) v = {spins[((size_t)(r0 + q) * N + k) * 2], spins[((size_t)(r0 + q) * N + k) * 2 + 1]};
+        s[q][k] = v;
+    }
+    __syncthreads();
+    double en[kEnRep];
+#pragma unroll
+    for (int q = 0; q < kEnRep; q++) en[q] = 0.0;
+    const int p1 = chunks[blockIdx.x + 1];
+    for (int p = chunks[blockIdx.x] + w; p < p1; p += kEnergyBS / 64) {
+        // pair (k < l) in colex order: p = C(l,2) + k (wave-uniform)
+        int l = (int)(0.5 + sqrt(0.25 + 2.0 * (double)p));
+        while (l * (l - 1) / 2 > p) l--;
+        while ((l + 1) * l / 2 <= p) l++;
+        const int k = p - l * (l - 1) / 2;
+        const int n = k * (k - 1) / 2;  // quartets i < j < k of the run
+        if (n == 0) continue;
+        const long long base = (long long)l * (l - 1) * (l - 2) * (l - 3) / 24 + (long long)k * (k - 1) * (k - 2) / 6;
+        c64 qkl[kEnRep], rkl[kEnRep], skl[kEnRep];
+#pragma unroll
+        for (int q = 0; q < kEnRep; q++) {
+            const c64 ak = s[q][k], al = s[q][l];
+            qkl[q] = cc(cm(ak, al));
+            rkl[q] = cm(cc(ak), al);
+            skl[q] = cm(ak, cc(al));
+        }
+        for (int r = lane; r < n; r += 64) {
+            int j = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)r)) * 0.5f);
+            if (j * (j - 1) / 2 > r) j--;
+            else if ((j + 1) * j / 2 <= r) j++;
+            const int i = r - j * (j - 1) / 2;
+            const uint8_t m = mask[base + r];
+            const double g = g4[base + r];
+            if (m == 0 || g == 0.0) continue;
+            const int ch = (m & 1) ? 0 : ((m & 2) ? 1 : 2);
+#pragma unroll
+            for (int q = 0; q < kEnRep; q++) {
+                const c64 aj = s[q][j], ai = s[q][i];
+                const c64 u = ch == 0 ? cm(aj, qkl[q]) : cm(cc(aj), ch == 1 ? rkl[q] : skl[q]);
+                en[q] -= g * (ai.x * u.x - ai.y * u.y);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kEnRep; q++) {
+        double v = en[q];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[w][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kEnRep && r0 + (int)threadIdx.x < nrep) {
+        double v = 0.0;
+        for (int x = 0; x < kEnergyBS / 64; x++) v += red[x][threadIdx.x];
+        part[(size_t)blockIdx.x * nrep + r0 + threadIdx.x] = v;
+    }
+}
+
+// Per replica: the H2 part and the H4 partials in chunk order.
+__global__ __launch_bounds__(kEnergyBS) void k_energy_dense_fin(const double* __restrict__ spins,
+                                                                const double* __restrict__ g2, int h2_active, int N,
+                                                                const double* __restrict__ part, int nchunk, int nrep,
+                                                                double* __restrict__ e2, double* __restrict__ e4) {
     __shared__ c64 s[kMaxN];
     __shared__ double red[kEnergyBS];
-    int rep = blockIdx.x;
+    const int rep = blockIdx.x;
     const double* sp = spins + (size_t)2 * N * rep;
     for (int i = threadIdx.x; i < N; i += blockDim.x) s[i] = {sp[2 * i], sp[2 * i + 1]};
     __syncthreads();
     double eh2 = h2_active ? h2_part(s, g2, N) : 0.0;
-    double eh4 = 0.0;
-    long long npl = (long long)N * (N - 1) / 2;
-    for (long long p = threadIdx.x; p < npl; p += blockDim.x) {
-        // pair (k<l) in colex order: p = C(l,2) + k
-        int l = (int)(0.5 + sqrt(0.25 + 2.0 * (double)p));
-        while ((long long)l * (l - 1) / 2 > p) l--;
-        while ((long long)(l + 1) * l / 2 <= p) l++;
-        int k = (int)(p - (long long)l * (l - 1) / 2);
-        if (k < 2) continue;
-        long long base = (long long)l * (l - 1) * (l - 2) * (l - 3) / 24 + (long long)k * (k - 1) * (k - 2) / 6;
-        c64 ak = s[k], al = s[l];
-        long long q = base;
-        for (int j = 1; j < k; j++) {
-            c64 aj = s[j];
-            for (int i = 0; i < j; i++, q++) {
-                uint8_t m = mask[q];
-                double g = g4[q];
-                if (m == 0 || g == 0.0) continue;
-                int ch = (m & 1) ? 0 : ((m & 2) ? 1 : 2);
-                eh4 += -g * quartet_re(s[i], aj, ak, al, ch);
-            }
-        }
-    }
     eh2 = block_sum<kEnergyBS>(eh2, red);
-    eh4 = block_sum<kEnergyBS>(eh4, red);
     if (threadIdx.x == 0) {
+        double eh4 = 0.0;
+        for (int c = 0; c < nchunk; c++) eh4 += part[(size_t)c * nrep + rep];
         e2[rep] = eh2;
         e4[rep] = eh4;
     }
@@ -132,9 +190,35 @@ __global__ void k_add(const double* a, const double* b, double* out, int n) {
     if (i < n) out[i] = a[i] + b[i];
 }
 
+int energy_chunk_count(int N, int nrep) {
+    const int groups = (nrep + kEnRep - 1) / kEnRep, npairs = N * (N - 1) / 2;
+    return std::max(1, std::min(npairs, std::min(256, (2048 + groups - 1) / groups)));
+}
+
+std::vector<int> energy_chunks(int N, int nchunk) {
+    // pair ranges [c[x], c[x+1]) with about equal quartet counts C(k,2)
+    const int npairs = N * (N - 1) / 2;
+    long long total = 0;
+    for (int l = 1; l < N; l++)
+        for (int k = 0; k < l; k++) total += (long long)k * (k - 1) / 2;
+    std::vector<int> c(nchunk + 1, npairs);
+    c[0] = 0;
+    long long acc = 0;
+    int x = 1, p = 0;
+    for (int l = 1; l < N; l++)
+        for (int k = 0; k < l; k++, p++) {
+            while (x < nchunk && acc >= total * x / nchunk) c[x++] = p;
+            acc += (long long)k * (k - 1) / 2;
+        }
+    return c;
+}
+
 hipError_t launch_energy_dense(const pspin_state* st, double* e2, double* e4) {
-    hipLaunchKernelGGL(k_energy_dense, dim3(st->nrep), dim3(kEnergyBS), 0, st->stream, st->d_spins, st->d_g2_64,
-                       st->h2_active, st->d_g4_64, st->d_mask, st->N, e2, e4);
+    const int groups = (st->nrep + kEnRep - 1) / kEnRep;
+    hipLaunchKernelGGL(k_energy_dense_part, dim3(st->n_echunk, groups), dim3(kEnergyBS), 0, st->stream, st->d_spins,
+                       st->nrep, st->d_g4_64, st->d_mask, st->N, st->d_echunk, st->d_epart);
+    hipLaunchKernelGGL(k_energy_dense_fin, dim3(st->nrep), dim3(kEnergyBS), 0, st->stream, st->d_spins, st->d_g2_64,
+                       st->h2_active, st->N, st->d_epart, st->n_echunk, st->nrep, e2, e4);
     return hipGetLastError();
 }
 

@@ -234,6 +234,9 @@ struct pspin_state {
     uint64_t graph_gen = 0;  // bumped whenever captured sweep graphs become stale
     double* d_e2 = nullptr;
     double* d_e4 = nullptr;
+    int* d_echunk = nullptr;    // dense energy: pair-chunk bounds [n_echunk + 1]
+    double* d_epart = nullptr;  // dense energy: H4 partial sums [n_echunk][nrep]
+    int n_echunk = 0;
 
     // parallel tempering
     int pt_NT = 0, pt_nrep = 0;
@@ -279,6 +282,8 @@ hipError_t launch_scale_shift(double* g, long long n, double sigma, double mean,
 hipError_t launch_to_f32(const double* src, float* dst, long long n, hipStream_t s);
 hipError_t launch_init_spins(double* spins, int N, int nrep, const uint64_t* seeds, int cube, hipStream_t s);
 
+int energy_chunk_count(int N, int nrep);              // pair chunks of the dense energy kernel
+std::vector<int> energy_chunks(int N, int nchunk);    // their bounds, about equal quartet counts
 hipError_t launch_energy_dense(const pspin_state* st, double* e2, double* e4);
 hipError_t launch_energy_sparse(const pspin_state* st, double* e2, double* e4);
 hipError_t launch_add(const double* a, const double* b, double* out, int n, hipStream_t s);
