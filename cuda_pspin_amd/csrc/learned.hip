@@ -304,8 +304,11 @@ __global__ __launch_bounds__(kEnThreads) void k_lp_energy(const double* __restri
 #pragma unroll
     for (int q = 0; q < kEnCfg; q++) en[q] = 0.0;
     uint32_t ckl = 0xFFFFFFFFu, cj = 0xFFFFFFFFu;
+    // the next term is read while this one is used (one memory latency per chunk, not per term)
+    uint4 wn = t0 < t1 ? terms[t0] : make_uint4(0u, 0u, 0u, 0u);
     for (long long t = t0; t < t1; t++) {
-        const uint4 w = terms[t];
+        const uint4 w = wn;
+        if (t + 1 < t1) wn = terms[t + 1];
         const double g = __hiloint2double((int)w.w, (int)w.z);
         const uint32_t i = w.x & 255u, j = (w.x >> 8) & 255u;
         if (w.y == 3u) {
