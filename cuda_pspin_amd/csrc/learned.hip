@@ -263,7 +263,7 @@ __device__ __forceinline__ c64d ccj(c64d p) { return {p.x, -p.y}; }
 
 // Full FP64 energies of 2R configurations (current states, then proposals).
 // A workgroup takes kEnCfg configurations (amplitudes in LDS as
-// [site][config]) and a range of the term list; each thread walks kEnChunk
+// [config][site]) and a range of the term list; each thread walks kEnChunk
 // consecutive terms for all kEnCfg configurations.  The list is in colex
 // order (l, k, j outer to inner), so a thread keeps the products of a_k, a_l
 // and of a_j with them across its terms and a quartet term costs one
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kEnThreads) void k_lp_energy(const double* __restri
                                                           const double* __restrict__ prop, int R, int N,
                                                           const uint4* __restrict__ terms, long long nt,
                                                           double* __restrict__ part) {
-    __shared__ c64d sl[kLearnedMaxN][kEnCfg];
+    __shared__ c64d sl[kEnCfg][kLearnedMaxN + 1];  // [config][site]
     __shared__ double red[kEnThreads / 64][kEnCfg];
     const int c0 = blockIdx.x * kEnCfg;
     for (int e = threadIdx.x; e < kEnCfg * N; e += kEnThreads) {
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kEnThreads) void k_lp_energy(const double* __restri
             const double* src = c < R ? cur + (size_t)2 * N * c : prop + (size_t)2 * N * (c - R);
             v = {src[2 * k], src[2 * k + 1]};
         }
-        sl[k][q] = v;
+        sl[q][k] = v;
     }
     __syncthreads();
     const long long per_block = (long long)kEnThreads * kEnChunk;
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(kEnThreads) void k_lp_energy(const double* __restri
         if (w.y == 3u) {
 #pragma unroll
             for (int q = 0; q < kEnCfg; q++) {
-                const c64d ai = sl[i][q], aj = sl[j][q];
+                const c64d ai = sl[q][i], aj = sl[q][j];
                 en[q] -= g * (ai.x * aj.x + ai.y * aj.y);
             }
             continue;
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kEnThreads) void k_lp_energy(const double* __restri
         if (kl != ckl) {
 #pragma unroll
             for (int q = 0; q < kEnCfg; q++) {
-                const c64d ak = sl[kl & 255u][q], al = sl[kl >> 8][q];
+                const c64d ak = sl[q][kl & 255u], al = sl[q][kl >> 8];
                 qkl[q] = ccj(cmul(ak, al));
                 rkl[q] = cmul(ccj(ak), al);
                 skl[q] = cmul(ak, ccj(al));
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kEnThreads) void k_lp_energy(const double* __restri
         if (j != cj) {
 #pragma unroll
             for (int q = 0; q < kEnCfg; q++) {
-                const c64d aj = sl[j][q];
+                const c64d aj = sl[q][j];
                 const c64d u0 = cmul(aj, qkl[q]), u1 = cmul(ccj(aj), rkl[q]), u2 = cmul(ccj(aj), skl[q]);
                 p0x[q] = u0.x; p0y[q] = u0.y;
                 p1x[q] = u1.x; p1y[q] = u1.y;
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kEnThreads) void k_lp_energy(const double* __restri
         for (int q = 0; q < kEnCfg; q++) {
             const double px = c0sel ? p0x[q] : (c1sel ? p1x[q] : p2x[q]);
             const double py = c0sel ? p0y[q] : (c1sel ? p1y[q] : p2y[q]);
-            const c64d ai = sl[i][q];
+            const c64d ai = sl[q][i];
             en[q] -= g * (ai.x * px - ai.y * py);
         }
     }
