@@ -89,6 +89,9 @@ __device__ __forceinline__ float chain_sum(float v) {
 // the fields of site i over every (L/2)-th entry of its lists, lanes [L/2, L)
 // those of site j, and chain_sum adds the L partial Delta E.  Lane sub = 0
 // writes the accepted amplitudes and owns the chain's counters and energy.
+// The design point is one workgroup per CU (a sample at N = 64 with 96 chains
+// takes about 104 KB of LDS), so a SIMD holds TB / 256 of its waves and the
+// compiler may use the registers that leaves.
 template <int L, int TB>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256))) void k_batch(BatchArgs a) {
     extern __shared__ double lds[];
