@@ -118,3 +118,23 @@ def test_initial_spins_and_energy(pspin, oracle, variant):
         assert abs(E[r] - (e2 + e4)) <= 1e-12 * max(1, abs(e2 + e4))
     assert not acc.any() and not prop.any()
     st.close()
+
+
+@pytest.mark.parametrize("N,nrep,fmc,gamma", [(33, 10, 0, 0.0), (64, 6, 2, 0.05), (40, 9, 1, 1.0)])
+def test_dense_energies_match_oracle(pspin, oracle, N, nrep, fmc, gamma):
+    """FP64 full energies (workgroups of 4 replicas over balanced chunks of
+    colex runs; 10 and 9 replicas leave a partial group) against the oracle's
+    FP64 energy, after a few sweeps away from the initial state."""
+    print(f"[dense energies] N={N} nrep={nrep} fmc={fmc}", flush=True)
+    st, cfg = _mk(pspin, 0, N=N, nrep=nrep, seed=77, fmc_mode=fmc, gamma=gamma)
+    st.set_uniform_beta(1.0)
+    st.sweep(3)
+    z = st.spins()
+    e2g, e4g = st.split_energies()
+    g2 = st.g2()
+    g4, mask = st.g4()
+    for r in range(nrep):
+        e2, e4 = oracle.energy(N, z[r], g2, st.info.h2_active, g4, mask)
+        assert abs(e2g[r] - e2) <= 1e-12 * max(1, abs(e2)), (r, e2g[r], e2)
+        assert abs(e4g[r] - e4) <= 1e-12 * max(1, abs(e4)), (r, e4g[r], e4)
+    st.close()
