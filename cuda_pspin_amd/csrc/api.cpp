@@ -589,6 +589,12 @@ int pt_advance(pspin_state* st, int n_sweeps, int pt_freq) {
     }
     return PSPIN_OK;
 }
+int pt_check_barrier(pspin_state* st) {
+    unsigned int expired = 0;
+    PS_TRY_HIP(hipMemcpy(&expired, st->d_done + 2, sizeof(expired), hipMemcpyDeviceToHost));
+    PS_REQUIRE(!expired, PSPIN_ERR_HIP, "PT grid barrier wait expired: rounds ran without their exchange");
+    return PSPIN_OK;
+}
 }  // namespace ps
 
 extern "C" {
@@ -1039,10 +1045,7 @@ int pspin_pt_run(pspin_state* st, int n_sweeps, int pt_freq) {
     const int rc = ps::pt_advance(st, n_sweeps, pt_freq);
     if (rc) return rc;
     PS_TRY_HIP(hipStreamSynchronize(st->stream));
-    unsigned int expired = 0;
-    PS_TRY_HIP(hipMemcpy(&expired, st->d_done + 2, sizeof(expired), hipMemcpyDeviceToHost));
-    PS_REQUIRE(!expired, PSPIN_ERR_HIP, "PT grid barrier wait expired: rounds ran without their exchange");
-    return PSPIN_OK;
+    return ps::pt_check_barrier(st);
 }
 
 int pspin_pt_exchange(pspin_state* st) {

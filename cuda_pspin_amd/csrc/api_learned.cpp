@@ -597,7 +597,7 @@ int pspin_lp_run(pspin_learned* lp, long long n_sweeps, int pt_freq, int move_ev
         done += chunk;
     }
     LP_TRY_HIP(sync(lp));
-    return PSPIN_OK;
+    return ps::pt_check_barrier(st);
 }
 
 int pspin_lp_get_stats(const pspin_learned* lp, long long* acc, long long* prop) {

@@ -34,9 +34,10 @@ __device__ __forceinline__ bool accept_below(double u, double delta) {
 
 // Exchange data crosses workgroups inside one launch (energies written by
 // every workgroup, the permutation and betas by the exchanging one), so it
-// moves with device-scope relaxed atomics: coherent across the XCDs' L2s
-// with no cache write-back or invalidation.  Ordering comes from the
-// arrival and release atomics after a workgroup fence.
+// moves with agent-scope atomic loads and stores, which no XCD serves from a
+// stale cache line.  Ordering comes from agent-scope release RMWs (arrival,
+// generation bump) and acquire fences (last arriver, released waiters) in
+// the sweep launch's grid barrier.
 template <typename T>
 __device__ __forceinline__ T dev_load(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

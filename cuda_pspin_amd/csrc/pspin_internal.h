@@ -335,4 +335,7 @@ hipError_t launch_exchange(pspin_state* st);
 
 // pspin_pt_run without the final stream synchronisation (api.cpp).
 int pt_advance(pspin_state* st, int n_sweeps, int pt_freq);
+// After the stream has drained: PSPIN_ERR_HIP if a grid-barrier wait of a
+// cooperative PT launch expired (rounds then ran without their exchange).
+int pt_check_barrier(pspin_state* st);
 }  // namespace ps

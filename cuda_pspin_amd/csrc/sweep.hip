@@ -663,14 +663,21 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         // Last workgroup out runs the replica exchange on every energy.  Between
         // rounds of a cooperative launch the others wait for it (a grid barrier:
         // arrival count, then a generation word bumped after the exchange).
-        // The data crossing workgroups moves by device-scope atomics (see
-        // dev_load), so a workgroup fence (the store has landed) orders it.
+        // Ordering across workgroups follows the HIP memory model at agent
+        // scope: each arrival is a release RMW (so its energy store, and its
+        // read of the generation, come before it), the last arriver acquires
+        // before staging the energies; the generation bump is a release RMW
+        // after the exchange's stores (made ordered before it by the
+        // workgroup barrier), and a waiter acquires once it sees the bump,
+        // before reading its new beta.  The data itself moves by agent-scope
+        // atomics (dev_load, dev_store), so no access is served from a stale
+        // cache line.
         const bool more = rd + 1 < a.rounds;
         if (tid == 0) {
             dev_store(a.E + rep, E);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             gen_seen = dev_load(a.done + 1);
-            last_wg = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+            last_wg = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+            if (last_wg) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         if (last_wg) {
@@ -679,9 +686,8 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             __syncthreads();
             if (w == 0) exchange_wave(a.ex, &view[0][0], lane);
             if (more) {
-                if (w == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                if (tid == 0) __hip_atomic_fetch_add(a.done + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __syncthreads();
+                __syncthreads();  // wave 0's exchange stores happen before the release below
+                if (tid == 0) __hip_atomic_fetch_add(a.done + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 // the exchange staged its slots over the amplitude copies
                 for (int i = lane; i < ((N + 3) & ~3); i += 64)
                     s32[i] = i < N ? make_float2((float)s64[i].x, (float)s64[i].y) : make_float2(0.f, 0.f);
@@ -697,6 +703,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                     }
                     __builtin_amdgcn_s_sleep(2);
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
         }
         if (more) {

@@ -129,6 +129,47 @@ def test_launch_modes_give_identical_chains(pspin, oracle, N, NT, nr, kernel):
         assert a[4] == b[4] and a[5] == b[5] == 136
 
 
+def test_cooperative_rounds_match_kernel_boundary_exchange(pspin, oracle):
+    # 1024 PT rounds at the N=64 benchmark shape (fully connected, 32 T x 8
+    # ladders, pair-row kernel, ladder-shared pair streams): the cooperative
+    # launch, whose rounds meet at an in-kernel grid barrier, against direct
+    # launches with the exchange in a kernel of its own (profiling on), where
+    # every handoff is ordered by a kernel boundary.  The chains must agree
+    # bit for bit after every block of 64 rounds; the first block that
+    # differs names the round range.
+    N, NT, nr = 64, 32, 8
+    T, beta = oracle.temperatures(NT, 0.2, 1.6, 1)
+    sts = []
+    for mode, prof in ((pspin.LAUNCH_AUTO, False), (pspin.LAUNCH_DIRECT, True)):
+        cfg = pspin.SimConfig.default(N=N, nrep=NT * nr, seed=42)
+        st = pspin.PSpin(cfg, 0)
+        st.set_kernel(pspin.KERNEL_PAIRS, 0)
+        st.set_pair_schedule(8)
+        st.pt_init(NT, nr, beta, 42 + 5000)
+        st.set_launch_mode(mode)
+        st.set_profiling(prof)
+        sts.append(st)
+    block, nblocks = 64, 16
+    for b in range(nblocks):
+        snap = []
+        for st in sts:
+            st.pt_run(block, 1)
+            E, acc, prop = st.results()
+            snap.append((st.spins(), E, acc, st.pt_perm(), st.pt_rng48(), st.pt_exchange_stats()))
+        a, c = snap
+        where = f"rounds {b * block}..{(b + 1) * block - 1}"
+        np.testing.assert_array_equal(a[3], c[3], err_msg=where)
+        np.testing.assert_array_equal(a[1], c[1], err_msg=where)
+        np.testing.assert_array_equal(a[0], c[0], err_msg=where)
+        np.testing.assert_array_equal(a[2], c[2], err_msg=where)
+        assert a[4] == c[4], where
+        for x, y in zip(a[5], c[5]):
+            np.testing.assert_array_equal(x, y, err_msg=where)
+    assert sts[1].profile()["n_sweep"] == block * nblocks
+    for st in sts:
+        st.close()
+
+
 def test_periodic_resync(pspin, oracle):
     cfg = pspin.SimConfig.default(N=14, nrep=4, seed=77)
     st = pspin.PSpin(cfg, 0)
