@@ -1,17 +1,22 @@
 """Throughput benchmark: dense spherical parallel tempering on one MI355X.
 
-Workload (the reference's `parallel_tempering -N 64 -NT 32 -nrep 8 -Tmin 0.2
--Tmax 1.6 -log_temp -pt_freq 1 -fmc 0 -seed 42`): N=64 fully connected
-(C(64,4) = 635,376 quartets, real couplings, one channel each), 32 geometric
-temperatures x 8 ladders = 256 replicas, an exchange after every sweep.
-A step is one sweep of all 256 replicas (N/2 = 32 pair proposals each) plus
-its exchange round, all on the GPU.
+Headline workload (the reference's `parallel_tempering -N 64 -NT 32 -nrep 8
+-Tmin 0.2 -Tmax 1.6 -log_temp -pt_freq 1 -fmc 0 -seed 42`): N=64 fully
+connected (C(64,4) = 635,376 quartets, real couplings, one channel each), 32
+geometric temperatures x 8 ladders = 256 replicas, an exchange after every
+sweep.  A step is one production launch: a cooperative launch of 32 PT rounds
+(a sweep of all 256 replicas, N/2 = 32 pair proposals each, then the
+exchange), all on the GPU.
 
-Reports one JSON line: pair-updates/s (whole job, timed by wall clock around
-K steps after W warm-up steps, inputs resident in HBM), energy ESS/s at the
-coldest slot, the sweep kernel's roofline figure (HIP events around every
-sweep launch on the library's stream) and a CPU baseline (the oracle's
-OpenMP transcription of the reference sweep, FP64) on a bounded sample.
+Reports one JSON line: pair-updates/s (whole job, wall clock around exactly K
+steps after W warm-up steps, inputs resident in HBM, no profiling), the sweep
+kernel's roofline (HIP events around every launch on the library's stream, in
+a separate pass of the same launch path), energy ESS/s at the coldest slot
+from a 2^16-sweep series, a CPU baseline (the oracle's OpenMP transcription of
+the reference sweep, FP64) on a bounded sample, the reference's per-replica
+pair schedule, the other configurations (N=32 uniform FMC, N=128 comb with
+BF16 hi/lo couplings and its Delta E error against FP64, the batched sparse
+workload) and the learned-vs-plain comparison.
 """
 import argparse
 import json
@@ -26,6 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+FP32_PEAK = 157.3e12  # MI355X FP32, flop/s
+LDS_PEAK = 128 * 256 * 2.4e9  # 128 B/clk per CU, 256 CUs, 2.4 GHz
+ROUNDS = 32  # PT rounds per cooperative launch (the library's production launch)
 
 
 def n_inv(N):
@@ -56,20 +64,6 @@ def tau_int_sokal(x, c=5.0):
         if M >= c * tau:
             break
     return max(tau, 1.0)
-
-
-def ess_of_series(series):
-    """series [rounds, ladders]: drop the first half, sum n/tau over ladders."""
-    half = series.shape[0] // 2
-    prod = series[half:]
-    ess = 0.0
-    taus = []
-    for r in range(prod.shape[1]):
-        t = tau_int_sokal(prod[:, r])
-        taus.append(t)
-        if np.isfinite(t):
-            ess += prod.shape[0] / t
-    return ess, taus
 
 
 def ess_blocks(series):
@@ -185,30 +179,44 @@ def learned_vs_plain(args):
     return out
 
 
-def cpu_baseline(st, N, nrep, betas, seed, n_sweeps):
+def cpu_baseline(st, N, NT, nr, beta_sched, n_warm=16, n_sweeps=256):
+    """The oracle's OpenMP transcription of the reference sweep (FP64, the
+    reference's 3-type enumeration with its index decode), one replica chain
+    per thread, on a bounded sample: as many replicas of the benchmark state
+    as there are cores, spread over the temperature slots, n_warm untimed
+    sweeps and then n_sweeps timed ones."""
     import oracle as O
 
     O.build()
     L = O.lib()
-    g2 = np.ascontiguousarray(st.g2(), dtype=np.float64).reshape(-1)
-    g4, mask = st.g4()
-    s = st.spins()
-    spins = np.ascontiguousarray(np.stack([s.real, s.imag], -1), dtype=np.float64).reshape(-1)
-    seeds, _ = st.rng_info()
-    E = np.zeros(nrep)
-    acc = np.zeros(nrep, dtype=np.int64)
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
     if os.environ.get("OMP_NUM_THREADS"):
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
-    secs = L.or_cpu_baseline_sweeps(N, nrep, spins, g2, int(st.info.h2_active), g4, mask,
-                                    np.ascontiguousarray(betas), seeds, 1 << 40, n_sweeps, E, acc, cores)
-    pu = n_sweeps * nrep * (N // 2)
+    R = NT * nr
+    n = max(1, min(cores, R))
+    perm = st.pt_perm()
+    slots = np.linspace(0, R - 1, n).round().astype(int)
+    reps = perm[slots]
+    betas = np.ascontiguousarray([beta_sched[s // nr] for s in slots], dtype=np.float64)
+    g2 = np.ascontiguousarray(st.g2(), dtype=np.float64).reshape(-1)
+    g4, mask = st.g4()
+    s = st.spins()[reps]
+    spins = np.ascontiguousarray(np.stack([s.real, s.imag], -1), dtype=np.float64).reshape(-1)
+    seeds = np.ascontiguousarray(st.rng_info()[0][reps])
+    E = np.zeros(n)
+    acc = np.zeros(n, dtype=np.int64)
+    h2 = int(st.info.h2_active)
+    L.or_cpu_baseline_sweeps(N, n, spins, g2, h2, g4, mask, betas, seeds, 1 << 40, n_warm, E, acc, cores)
+    secs = L.or_cpu_baseline_sweeps(N, n, spins, g2, h2, g4, mask, betas, seeds, (1 << 40) + n_warm, n_sweeps, E,
+                                    acc, cores)
+    pu = n_sweeps * n * (N // 2)
     return {"value": pu / secs, "unit": "pair-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{n_sweeps} sweep(s) x {nrep} replicas of the same N={N} system "
-                      f"({pu} pair-updates, {secs:.1f} s, FP64, reference 3-type enumeration)"}
+            "sample": f"{n} replicas of the benchmark state (one per thread, spread over the {NT} temperatures), "
+                      f"{n_warm} warm-up sweeps then {n_sweeps} timed sweeps: {pu} pair-updates in {secs:.1f} s, "
+                      "FP64, reference 3-type enumeration, no exchange"}
 
 
 def copy_calibration(nbytes):
@@ -230,31 +238,16 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 4}))
 
 
-def load_traffic(N, fmc, kernel):
-    """HBM bytes per sweep launch measured with rocprofv3 --pmc (profiles/), or None."""
-    return load_traffic_key(f"N{N}_fmc{fmc}_k{kernel}")
-
-
-def load_traffic_key(key):
+def load_traffic(key):
+    """HBM bytes per sweep launch of the named workload, measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the copy calibration
+    (profiles/traffic.json), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
-    if not os.path.exists(path):
-        return None
     try:
         rec = json.load(open(path))
-    except ValueError:
-        return None
-    return rec.get(key, {}).get("hbm_bytes_per_launch")
-
-
-def batch_traffic(sweeps_per_launch):
-    """HBM bytes of one batch launch of sweeps_per_launch sweeps, from the
-    measured bytes per sweep (the per-sweep draw scratch dominates)."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
-    try:
-        per = json.load(open(path)).get("batch_256", {}).get("hbm_bytes_per_sweep")
     except (OSError, ValueError):
         return None
-    return None if per is None else per * sweeps_per_launch
+    return rec.get(key, {}).get("hbm_bytes_per_launch")
 
 
 def splitmix64(state):
@@ -264,74 +257,241 @@ def splitmix64(state):
     return z ^ (z >> 31)
 
 
-def run_batch(args):
+def flops_per_pair_update(N, n_quart_active, h2_active):
+    """FP32 flops of one pair-update in the reference's algorithm: 24 per
+    active quartet holding i0 or j0 (three complex products of the
+    differential and the Re(g .) multiply-add, src/mc.cu:182-275), i.e.
+    n_inv(N) 24 when every quartet is active, plus 8 per g2 entry of the two
+    rows (src/mc.cu:160-180)."""
+    c4 = N * (N - 1) * (N - 2) * (N - 3) // 24
+    return 24.0 * n_quart_active * n_inv(N) / c4 + (2 * N * 4 if h2_active else 0)
+
+
+def geometric_betas(NT, Tmin, Tmax):
+    A = (Tmin / Tmax) ** (1.0 / (NT - 1))
+    T = np.array([Tmax * A ** t for t in range(NT)])
+    return T, 1.0 / T
+
+
+def dense_pt(P, N, NT, nr, Tmin, Tmax, fmc, gamma, coupling, pair_groups, seed, steps, warmup, prof_launches):
+    """Dense PT on one state: autotuned sweep kernel, W warm-up steps, then
+    EXACTLY K timed steps, a step being one production launch: a cooperative
+    launch of ROUNDS PT rounds (a sweep of every replica, then the exchange),
+    with no profiling.  A separate pass of prof_launches launches records HIP
+    events around every launch on the library's stream (the same launch
+    path) for the kernel's duration.  Returns the state and the figures."""
+    R = NT * nr
+    T, beta = geometric_betas(NT, Tmin, Tmax)
+    cfg = P.SimConfig.default(N=N, nrep=R, seed=seed, fmc_mode=fmc, gamma=gamma, dev=0)
+    t0 = time.time()
+    st = P.PSpin(cfg, P.DENSE)
+    if coupling == "bf16":
+        st.set_coupling_format(1)
+    st.set_pair_schedule(pair_groups)
+    tuned = st.autotune(8)  # times every kernel / tile candidate on this state and restores it
+    st.pt_init(NT, nr, beta, seed + 5000)
+    t_init = time.time() - t0
+    st.pt_run(warmup * ROUNDS, 1)
+    t0 = time.perf_counter()
+    st.pt_run(steps * ROUNDS, 1)  # synchronises before returning
+    wall = time.perf_counter() - t0
+    st.set_profiling(True)
+    st.pt_run(prof_launches * ROUNDS, 1)
+    prof = st.profile()
+    st.set_profiling(False)
+    launch_ms = prof["sweep_ms"] / max(prof["n_launch"], 1)
+    per_launch = prof["n_sweep"] / max(prof["n_launch"], 1)
+    pu_sweep = R * (N // 2)
+    fpu = flops_per_pair_update(N, st.info.n_quart_active, st.info.h2_active)
+    achieved = pu_sweep * per_launch * fpu / (launch_ms * 1e-3)
+    out = {"value": steps * ROUNDS * pu_sweep / wall, "unit": "pair-updates/s", "steps": steps, "warmup": warmup,
+           "ms_per_step": wall / steps * 1e3, "ms_per_sweep": wall / (steps * ROUNDS) * 1e3,
+           "config": {"N": N, "NT": NT, "nrep": nr, "replicas": R, "T": [Tmin, Tmax], "pt_freq": 1, "fmc": fmc,
+                      "gamma": gamma, "coupling": coupling, "pair_groups": pair_groups, "seed": seed,
+                      "rounds_per_step": ROUNDS, "sweeps_timed": steps * ROUNDS,
+                      "quartets_active": st.info.n_quart_active, "h2_active": st.info.h2_active,
+                      "autotuned": {"kernel": tuned[0], "block": tuned[1], "trial_ms_per_sweep": tuned[2]}},
+           "init_s": t_init,
+           "sweep_launch": {"path": "cooperative" if per_launch == ROUNDS else "direct", "sweeps_per_launch": per_launch,
+                            "launch_ms": launch_ms, "launches_timed": prof["n_launch"]},
+           "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                        "frac": achieved / FP32_PEAK,
+                        "traffic": load_traffic(f"N{N}_fmc{fmc}_k{st.info.kernel}_{coupling}"),
+                        "flops_per_pair_update": fpu, "pair_updates_per_launch": int(pu_sweep * per_launch),
+                        "launch_ms": launch_ms,
+                        "hbm_algorithmic_bytes_per_pair_update": bytes_per_pair_update(N, st.info.h2_active)
+                        if fmc == 0 else None}}
+    return st, out
+
+
+def energy_ess(st, nr, burn, prod):
+    """Energy series of the coldest slot (the replica holding T_min after each
+    exchange), `burn` sweeps dropped, `prod` recorded; per ladder
+    ESS = prod / tau_int (Sokal window, c = 5), summed over ladders, over the
+    wall time of the recorded sweeps.  Null unless every ladder's series spans
+    at least 50 tau_int."""
+    st.pt_run(burn, 1)
+    st.pt_enable_series(prod)
+    t0 = time.perf_counter()
+    st.pt_run(prod, 1)
+    w = time.perf_counter() - t0
+    ser = st.pt_series()
+    taus = np.array([tau_int_sokal(ser[:, r]) for r in range(nr)])
+    per = ser.shape[0] / taus
+    ok = bool(np.all(np.isfinite(taus)) and per.min() >= 50)
+    ess = float(per.sum())
+    return {"energy_ess_per_s_tmin": ess / w if ok else None,
+            "energy_ess_per_s_tmin_stderr": float(math.sqrt(nr) * per.std(ddof=1) / w) if ok and nr > 1 else None,
+            "tau_int_sweeps_by_ladder": [float(t) for t in taus],
+            "series_over_tau_min": float(per.min()), "burn_in_sweeps": burn, "recorded_sweeps": int(ser.shape[0]),
+            "recorded_wall_s": w, "tau_e_crossing_sweeps": tau_e_crossing(ser)}
+
+
+def delta_e_error(st, n_sweeps=2):
+    """FP32 Delta E of every proposal of n_sweeps logged sweeps against the
+    library's FP64 full energies of the configurations before and after the
+    proposal (E64(after) - E64(before); the FP64 difference is exact to
+    ~1e-13 here).  The replay applies the logged accepted amplitudes, and
+    must end on the kernel's own final spins."""
+    R, N = st.nrep, st.N
+    z = st.spins().copy()
+    st.enable_step_log(n_sweeps)
+    st.sweep(n_sweeps)
+    log, n = st.step_log()
+    z_end = st.spins()
+    steps = log.reshape(R, n * (N // 2))
+    rows = np.arange(R)
+
+    def energies(zz):
+        st.set_spins(zz)
+        e2, e4 = st.split_energies()
+        return e2 + e4
+
+    e_cur = energies(z)
+    err, d64 = [], []
+    for k in range(steps.shape[1]):
+        rec = steps[:, k]
+        zp = z.copy()
+        zp[rows, rec["i0"]] = rec["ai"][:, 0] + 1j * rec["ai"][:, 1]
+        zp[rows, rec["j0"]] = rec["aj"][:, 0] + 1j * rec["aj"][:, 1]
+        e_p = energies(zp)
+        d = e_p - e_cur
+        err.append(rec["dE"].astype(np.float64) - d)
+        d64.append(d)
+        acc = rec["acc"].astype(bool)
+        z[acc] = zp[acc]
+        e_cur = np.where(acc, e_p, e_cur)
+    err, d64 = np.abs(np.concatenate(err)), np.concatenate(d64)
+    st.set_spins(z_end)
+    rms64 = float(np.sqrt(np.mean(d64 ** 2)))
+    return {"proposals": int(err.size), "max_abs": float(err.max()), "rms_abs": float(np.sqrt(np.mean(err ** 2))),
+            "rms_dE64": rms64, "max_abs_over_rms_dE64": float(err.max()) / rms64,
+            "rms_abs_over_rms_dE64": float(np.sqrt(np.mean(err ** 2))) / rms64,
+            "replay_ends_on_kernel_spins": bool(np.array_equal(z, z_end))}
+
+
+def run_batch(P, seed, steps, warmup, n_samples=256, chunk=0):
     """Batched sparse PT: 256 disorder samples, N_s uniform on {24,32,48,64}
     (splitmix64(seed+s) mod 4), cube constraint, uniform FMC gamma=0.05, N
     sampled quartets each, 24 geometric temperatures in [0.2,1.6] x 4 replicas,
-    FP64 resync every 1024 sweeps, one launch per 64 sweeps (graph-replayed)."""
-    import cuda_pspin_amd as P
-
-    n_samples, NT, nr = args.samples, 24, 4
-    Ns = [(24, 32, 48, 64)[splitmix64(args.seed + s) % 4] for s in range(n_samples)]
-    seeds = [args.seed + s for s in range(n_samples)]
-    T = np.array([1.6 * (0.2 / 1.6) ** (t / (NT - 1)) for t in range(NT)])
+    FP64 resync every 1024 sweeps, sweeps per launch autotuned, graph-replayed
+    launches.  A step is one sweep of the whole batch plus its exchanges."""
+    NT, nr = 24, 4
+    Ns = [(24, 32, 48, 64)[splitmix64(seed + s) % 4] for s in range(n_samples)]
+    seeds = [seed + s for s in range(n_samples)]
+    T, beta = geometric_betas(NT, 0.2, 1.6)
     base = P.SimConfig.default(fmc_mode=2, gamma=0.05, dev=0)
     t0 = time.time()
-    b = P.PSpinBatch(base, Ns, seeds, NT, nr, 1.0 / T)
+    b = P.PSpinBatch(base, Ns, seeds, NT, nr, beta)
     t_init = time.time() - t0
     b.set_resync_interval(1024)
-    b.run(args.warmup, 1)
-    chunk, trial_ms = b.autotune(512, 1)  # sweeps per launch; restores the batch afterwards
+    b.run(warmup, 1)
+    if chunk:
+        b.set_chunk(chunk)
+        trial_ms = None
+    else:
+        chunk, trial_ms = b.autotune(512, 1)  # sweeps per launch; restores the batch afterwards
     t0 = time.perf_counter()
-    b.run(args.steps, 1)
+    b.run(steps, 1)
     wall = time.perf_counter() - t0
-    pu = b.pair_updates_per_sweep * args.steps
-    # algorithmic bytes of a sparse pair-update: the N stored quartet terms
-    # scanned (16 B each: indices + FP64 coupling) plus two g2 rows (8 B each)
-    bpu = np.mean([16 * n + 2 * n * 8 for n in Ns])
+    pu = b.pair_updates_per_sweep * steps
+    # LDS bytes a pair-update must read: the two sites' H4 entries (4 per site
+    # on average: N quartets x 4 sites / N; 8 B each) with three FP64 complex
+    # amplitudes each (48 B), and their H2 entries (2 gamma (N-1) partners on
+    # average; 8 B entry + one 16 B amplitude)
+    bpu = float(np.mean([2 * (4 * (8 + 48) + 2 * 0.05 * (n - 1) * (8 + 16)) for n in Ns]))
     achieved = pu / wall * bpu
-    out = {"metric": "pair-updates/s (batched sparse PT, 256 samples, 1 MI355X)", "value": pu / wall,
-           "unit": "pair-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded disorder and spins)",
+    traffic = load_traffic("batch_256")
+    out = {"value": pu / wall, "unit": "pair-updates/s", "steps": steps, "warmup": warmup,
+           "ms_per_step": wall / steps * 1e3,
            "config": {"workload": "batched sparse cube PT: 256 samples N in {24,32,48,64}, fmc uniform 0.05, "
-                                  "NT=24 x nrep=4, exchange every sweep, resync 1024",
+                                  "NT=24 x nrep=4, exchange every sweep, FP64 resync every 1024 sweeps",
                       "samples": n_samples, "chains": b.chains, "N_hist": {str(n): Ns.count(n) for n in (24, 32, 48, 64)},
                       "autotuned_sweeps_per_launch": chunk, "autotune_trial_ms_per_sweep": trial_ms},
            "init_s": t_init,
-           "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK, "traffic": batch_traffic(chunk),
-                        "note": "terms and amplitudes are LDS-resident; latency-bound per lane"}}
+           "roofline": {"bound": "lds", "achieved": achieved / 1e9, "peak": LDS_PEAK / 1e9, "unit": "GB/s",
+                        "frac": achieved / LDS_PEAK, "bytes_per_pair_update": bpu,
+                        "traffic": None if traffic is None else traffic / load_traffic_sweeps("batch_256") * chunk,
+                        "note": "terms and amplitudes are LDS-resident; the chain steps are VALU-issue-bound"}}
     b.close()
-    print(json.dumps(out))
+    return out
+
+
+def load_traffic_sweeps(key):
+    try:
+        return json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))[key]["sweeps_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return 1
+
+
+HEADLINE = "pair-updates/s + energy ESS/s at T_min, N=64 PT: learned vs plain, 1 MI355X"
+
+# The configurations besides the headline (cfg 2): N, NT, nrep, Tmin, Tmax,
+# fmc, gamma, coupling, iteration count of the reference invocation.
+CONFIGS = {
+    "cfg1": dict(N=32, NT=20, nr=4, Tmin=0.2, Tmax=1.6, fmc=2, gamma=0.05, coupling="f32", sweeps=1 << 14,
+                 workload="dense spherical PT N=32 fmc uniform gamma=0.05, NT=20 x nrep=4, T in [0.2,1.6] "
+                          "geometric, exchange every sweep, 2^14 sweeps"),
+    "cfg4": dict(N=128, NT=16, nr=4, Tmin=0.3, Tmax=1.6, fmc=1, gamma=1.0, coupling="bf16", sweeps=1 << 14,
+                 workload="dense spherical PT N=128 comb fmc gamma=1 (H2 off), BF16 hi/lo couplings, NT=16 x "
+                          "nrep=4, T in [0.3,1.6] geometric, exchange every sweep, 2^14 sweeps"),
+}
+
+
+def config_line(P, name, seed):
+    c = CONFIGS[name]
+    steps = c["sweeps"] // ROUNDS
+    st, out = dense_pt(P, c["N"], c["NT"], c["nr"], c["Tmin"], c["Tmax"], c["fmc"], c["gamma"], c["coupling"],
+                       c["nr"], seed, steps, 8, 16)
+    out["config"]["workload"] = c["workload"]
+    if name == "cfg4":
+        out["delta_e_vs_fp64"] = delta_e_error(st, 2)
+    st.close()
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16384)
-    ap.add_argument("--warmup", type=int, default=64)
-    ap.add_argument("--N", type=int, default=64)
-    ap.add_argument("--NT", type=int, default=32)
-    ap.add_argument("--nrep", type=int, default=8)
-    ap.add_argument("--fmc", type=int, default=0)
-    ap.add_argument("--gamma", type=float, default=0.0)
+    ap.add_argument("--steps", type=int, default=512, help="timed steps (one cooperative launch of 32 PT rounds each)")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--kernel", type=int, default=0)
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--coupling", choices=["f32", "bf16"], default="f32")
     ap.add_argument("--Tmin", type=float, default=0.2)
     ap.add_argument("--Tmax", type=float, default=1.6)
-    ap.add_argument("--cpu-sweeps", type=int, default=1)
+    ap.add_argument("--ess-sweeps", type=int, default=1 << 16,
+                    help="sweeps of the energy-ESS run (the first half is burn-in)")
+    ap.add_argument("--cpu-sweeps", type=int, default=256)
+    ap.add_argument("--cpu-warmup", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calib", type=int, default=0, help="only run the counter-calibration copy of this size")
-    ap.add_argument("--workload", choices=["dense", "batch", "learned"], default="dense")
-    ap.add_argument("--samples", type=int, default=256)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
-    ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (for tracing tools)")
-    ap.add_argument("--pair-groups", type=int, default=8,
-                    help="replicas r = g (mod G) share a pair stream, one group per XCD and per ladder "
-                         "(0: per-replica pairs, as the reference)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="headline only: skip the per-replica pair schedule, cfg 1 / 3 / 4 and the learned comparison")
+    ap.add_argument("--copy-calib", type=int, default=0, help="only run the counter-calibration copy of this size")
+    ap.add_argument("--workload", choices=["dense", "batch", "learned", "cfg1", "cfg4"], default="dense")
+    ap.add_argument("--no-graphs", action="store_true", help="learned workload: launch kernels one by one")
+    ap.add_argument("--batch-chunk", type=int, default=0,
+                    help="batch workload: sweeps per launch (0: autotuned); a fixed value gives every launch the "
+                         "same size, for counter runs")
     ap.add_argument("--lp-burn", type=int, default=8192)
     ap.add_argument("--lp-snapshots", type=int, default=8192)
     ap.add_argument("--lp-train-steps", type=int, default=1500)
@@ -348,90 +508,52 @@ def main():
               "(independent ladders would shard with no communication). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    import cuda_pspin_amd as P
+
     if args.workload == "batch":
-        run_batch(args)
+        print(json.dumps(run_batch(P, args.seed, args.steps * ROUNDS, args.warmup * ROUNDS, chunk=args.batch_chunk)))
         return
     if args.workload == "learned":
         print(json.dumps({"learned_vs_plain": learned_vs_plain(args)}))
         return
+    if args.workload in CONFIGS:
+        print(json.dumps(config_line(P, args.workload, args.seed)))
+        return
 
-    import cuda_pspin_amd as P
-
-    N, NT, nr = args.N, args.NT, args.nrep
-    nrep = NT * nr
-    Tmin, Tmax = args.Tmin, args.Tmax
-    A = (Tmin / Tmax) ** (1.0 / (NT - 1))
-    T = np.array([Tmax * A ** t for t in range(NT)])
-    beta = 1.0 / T
-    cfg = P.SimConfig.default(N=N, nrep=nrep, seed=args.seed, fmc_mode=args.fmc, gamma=args.gamma, dev=0)
-    t0 = time.time()
-    st = P.PSpin(cfg, P.DENSE)
-    if args.coupling == "bf16":
-        st.set_coupling_format(1)
-    st.set_pair_schedule(args.pair_groups)
-    tuned = None
-    if args.kernel == 0 and args.block == 0:
-        tuned = st.autotune(8)  # times every kernel / tile candidate on this state
+    # cfg 2: N=64 fully connected, 32 x 8, exchange every sweep, ladder-shared pair streams
+    N, NT, nr = 64, 32, 8
+    T, beta = geometric_betas(NT, args.Tmin, args.Tmax)
+    st, head = dense_pt(P, N, NT, nr, args.Tmin, args.Tmax, 0, 0.0, "f32", nr, args.seed, args.steps, args.warmup,
+                        min(args.steps, 64))
+    cfg = head["config"]
+    cfg["workload"] = (f"dense spherical PT N={N} fully connected (fmc 0), NT={NT} x nrep={nr}, T in "
+                       f"[{args.Tmin},{args.Tmax}] geometric, exchange every sweep; a step is one cooperative "
+                       f"launch of {ROUNDS} PT rounds")
+    out = {"metric": HEADLINE, "value": head["value"], "unit": "pair-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded disorder and spins)", "config": cfg,
+           "ms_per_sweep": head["ms_per_sweep"], "init_s": head["init_s"], "sweep_launch": head["sweep_launch"],
+           "roofline": head["roofline"]}
+    if args.ess_sweeps > 0:
+        ess = energy_ess(st, nr, args.ess_sweeps // 2, args.ess_sweeps - args.ess_sweeps // 2)
+        out["energy_ess_per_s_tmin"] = ess.pop("energy_ess_per_s_tmin")
+        out["energy_ess"] = ess
     else:
-        st.set_kernel(args.kernel, args.block)
-    t_init = time.time() - t0
-    st.pt_init(NT, nr, beta, args.seed + 5000)
-    st.pt_run(args.warmup, 1)
-    st.pt_enable_series(args.steps)
-    st.set_profiling(True)
-    t0 = time.perf_counter()
-    st.pt_run(args.steps, 1)  # synchronises before returning
-    wall = time.perf_counter() - t0
-    prof = st.profile()
-    st.set_profiling(False)
-    series = st.pt_series()
-    ess, taus = ess_of_series(series)
-
-    pu_per_step = nrep * (N // 2)
-    value = args.steps * pu_per_step / wall
-    kern_ms = prof["sweep_ms"] / max(prof["n_sweep"], 1)  # per sweep (exchange included when fused)
-    sweeps_per_launch = prof["n_sweep"] / max(prof["n_launch"], 1)
-    bpu = bytes_per_pair_update(N, st.info.h2_active)
-    if args.fmc != 0:
-        # FMC systems are stored sparsely: each surviving (site, term) entry is a
-        # 4-byte coupling plus 4 bytes of partner indices
-        bpu = 2.0 * st.info.n_site_entries / N * 8 + (2 * N * 4 if st.info.h2_active else 0)
-    achieved = pu_per_step * bpu / (kern_ms * 1e-3)
-    out = {
-        "metric": "pair-updates/s + energy ESS/s at T_min, N=64 PT: learned vs plain, 1 MI355X",
-        "value": value,
-        "unit": "pair-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded disorder and spins)",
-        "config": {"workload": f"dense spherical PT N={N} fmc={args.fmc} NT={NT} x nrep={nr} "
-                               f"T in [{Tmin},{Tmax}] geometric, exchange every sweep",
-                   "N": N, "NT": NT, "nrep": nr, "replicas": nrep, "pt_freq": 1, "fmc": args.fmc,
-                   "sweeps_timed": args.steps, "kernel": st.info.kernel, "coupling": args.coupling,
-                   "pair_groups": args.pair_groups,
-                   "autotuned": None if tuned is None else {"kernel": tuned[0], "block": tuned[1],
-                                                             "trial_ms_per_sweep": tuned[2]}},
-        "energy_ess_per_s_tmin": ess / wall,
-        "energy_tau_int_tmin_sweeps": float(np.nanmean(taus)) if taus else None,
-        "sweep_kernel_ms": kern_ms,
-        "sweeps_per_launch": sweeps_per_launch,
-        "exchange_kernel_ms": (prof["exch_ms"] / prof["n_exch"]) if prof["n_exch"] else None,
-        "init_s": t_init,
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": load_traffic(N, args.fmc, st.info.kernel),
-                     "bytes_per_pair_update": bpu,
-                     "pair_updates_per_launch": int(round(pu_per_step * sweeps_per_launch)),
-                     "launch_ms": kern_ms * sweeps_per_launch},
-    }
+        out["energy_ess_per_s_tmin"] = None
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(st, N, nrep, np.repeat(beta, nr), args.seed, args.cpu_sweeps)
+        out["cpu_baseline"] = cpu_baseline(st, N, NT, nr, beta, args.cpu_warmup, args.cpu_sweeps)
     st.close()
+    # the reference's schedule: every replica draws its own pairs (src/mc.cu:129-141)
+    if args.no_configs:
+        print(json.dumps(out))
+        return
+    st, ref = dense_pt(P, N, NT, nr, args.Tmin, args.Tmax, 0, 0.0, "f32", 0, args.seed, args.steps, args.warmup,
+                       min(args.steps, 16))
+    st.close()
+    out["per_replica_pairs"] = {k: ref[k] for k in ("value", "ms_per_step", "ms_per_sweep", "sweep_launch")}
+    out["per_replica_pairs"]["roofline_frac"] = ref["roofline"]["frac"]
+    out["other_configs"] = {name: config_line(P, name, args.seed) for name in CONFIGS}
+    out["other_configs"]["cfg3"] = run_batch(P, args.seed, 1 << 15, 256)
     if not args.no_learned:
         out["learned_vs_plain"] = learned_vs_plain(args)
     print(json.dumps(out))

@@ -16,7 +16,8 @@ import subprocess
 
 import numpy as np
 
-__all__ = ["SimConfig", "PSpin", "PSpinBatch", "load_library", "build", "DENSE", "SPARSE", "STEP_DTYPE", "PSpinError"]
+__all__ = ["SimConfig", "PSpin", "PSpinBatch", "load_library", "build", "call", "DENSE", "SPARSE", "STEP_DTYPE",
+           "PSpinError"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # CUDA_PSPIN_AMD_LIB selects another build of the same library (e.g. a
@@ -120,6 +121,20 @@ def load_library() -> C.CDLL:
         "pspin_get_g2": (C.c_int, [P, dp]),
         "pspin_get_g4": (C.c_int, [P, dp, u8p]),
         "pspin_get_sparse_quartets": (C.c_int, [P, i32p, dp]),
+        # free functions on device buffers (pointers as integers)
+        "pspin_generate_g2": (C.c_int, [P, C.c_int, C.c_double, C.c_ulonglong]),
+        "pspin_generate_g4": (C.c_int, [P, C.c_int, C.c_double, C.c_ulonglong]),
+        "pspin_rescale_g2": (C.c_int, [P, C.c_int, C.c_double, C.c_longlong]),
+        "pspin_rescale_g4": (C.c_int, [P, C.c_int, C.c_double, C.c_longlong]),
+        "pspin_shift_mean_couplings": (C.c_int, [P, C.c_longlong, C.c_double]),
+        "pspin_apply_fmc_g2": (C.c_int, [P, C.c_int, P, C.c_double]),
+        "pspin_init_g4_mask": (C.c_int, [P, C.c_int]),
+        "pspin_apply_fmc_g4": (C.c_int, [P, C.c_int, P, C.c_double]),
+        "pspin_count_active_g4": (C.c_int, [P, C.c_int, C.POINTER(C.c_longlong)]),
+        "pspin_compute_energy": (C.c_int, [P, P, P, P, C.c_int, C.POINTER(C.c_double)]),
+        "pspin_init_spins_uniform": (C.c_int, [P, C.c_int, C.c_ulonglong]),
+        "pspin_init_spins_cube": (C.c_int, [P, C.c_int, C.c_ulonglong]),
+        "pspin_decode_quartets": (C.c_int, [P, P, C.c_longlong]),
         "pspin_pt_init": (C.c_int, [P, C.c_int, C.c_int, dp, C.c_long]),
         "pspin_pt_run": (C.c_int, [P, C.c_int, C.c_int]),
         "pspin_pt_exchange": (C.c_int, [P]),
@@ -193,6 +208,12 @@ def _check(rc: int) -> None:
     if rc != 0:
         msg = load_library().pspin_last_error()
         raise PSpinError(f"pspin error {rc}: {msg.decode() if msg else ''}")
+
+
+def call(name: str, *args) -> None:
+    """Call C-ABI entry point `name` (device buffers as integer addresses)
+    and raise PSpinError when it fails."""
+    _check(getattr(load_library(), name)(*args))
 
 
 class PSpin:

@@ -138,6 +138,26 @@ PS_HD double lcg48_unit(uint64_t x) { return (double)x * (1.0 / 281474976710656.
 PS_HD long long n_quartets(int N) { return (long long)N * (N - 1) * (N - 2) * (N - 3) / 24; }
 PS_HD long long n_pairs(int N) { return (long long)N * (N + 1) / 2; }
 
+// Colex decode q -> (i<j<k<l), q = C(l,4)+C(k,3)+C(j,2)+i (src/disorder.cu:159-171).
+// Branch-light: l from a float estimate corrected by integer compares.
+__device__ __forceinline__ void decode_quartet(long long q, int& i, int& j, int& k, int& l) {
+    int L = (int)floor(pow(24.0 * (double)q, 0.25)) + 2;
+    while ((long long)L * (L - 1) * (L - 2) * (L - 3) / 24 > q) L--;
+    while ((long long)(L + 1) * L * (L - 1) * (L - 2) / 24 <= q) L++;
+    long long rem = q - (long long)L * (L - 1) * (L - 2) * (L - 3) / 24;
+    int K = (int)floor(cbrt(6.0 * (double)rem)) + 1;
+    if (K < 2) K = 2;
+    while ((long long)K * (K - 1) * (K - 2) / 6 > rem) K--;
+    while ((long long)(K + 1) * K * (K - 1) / 6 <= rem) K++;
+    rem -= (long long)K * (K - 1) * (K - 2) / 6;
+    int J = (int)floor(sqrt(2.0 * (double)rem)) + 1;
+    if (J < 1) J = 1;
+    while ((long long)J * (J - 1) / 2 > rem) J--;
+    while ((long long)(J + 1) * J / 2 <= rem) J++;
+    rem -= (long long)J * (J - 1) / 2;
+    i = (int)rem; j = J; k = K; l = L;
+}
+
 // Conjugation masks per channel (bit p set = sorted position p conjugated),
 // src/mc.cu:119-121.
 PS_HD int conj_mask(int ch) { return ch == 0 ? 0xC : (ch == 1 ? 0x6 : 0xA); }

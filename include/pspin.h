@@ -138,6 +138,65 @@ int pspin_get_g2(const pspin_state* st, double* g2);
 int pspin_get_g4(const pspin_state* st, double* g4, uint8_t* mask);
 int pspin_get_sparse_quartets(const pspin_state* st, int* quartets, double* gq);
 
+/* ---------------- free functions on caller-owned device buffers ----------
+ * The reference's disorder.h, hamiltonian.h, spins.h and spins_sparse.h API.
+ * Pointers named d_* are device pointers (hipMalloc, or a framework's
+ * tensor storage).  Couplings and spins are interleaved FP64 complex (the
+ * byte layout of cuDoubleComplex): g2 [N*N][2], g4 [C(N,4)][2], spins [N][2];
+ * masks are one byte per colex quartet q = C(l,4)+C(k,3)+C(j,2)+i; omega is
+ * FP64 [N].  Kernels run on the null stream and the call waits for them.
+ * The Gaussian values come from this library's Philox4x32-10 streams, not
+ * from cuRAND, so they match the reference in distribution only; applied in
+ * the reference's order (generate, FMC, count, rescale, shift) they equal
+ * what pspin_create builds for the same seed, bit for bit. */
+
+/* generate_g2 (include/disorder.h:23): unit real normals, symmetric, diagonal
+ * kept, imaginary parts zero.  J is unused, as in the reference. */
+int pspin_generate_g2(double* d_g2, int N, double J, unsigned long long seed);
+/* generate_g4 (include/disorder.h:29): one unit real normal per quartet
+ * (stream key seed + 12345, as src/disorder.cu:98). */
+int pspin_generate_g4(double* d_g4, int N, double J, unsigned long long seed);
+/* rescale_g2 / rescale_g4 (include/disorder.h:47,51): both parts times
+ * J sqrt(N / n_surviving); nothing when n_surviving <= 0. */
+int pspin_rescale_g2(double* d_g2, int N, double J, long long n_surviving);
+int pspin_rescale_g4(double* d_g4, int N, double J, long long n_surviving);
+/* shift_mean_couplings (include/disorder.h:55): real part + mean on every
+ * entry with a non-zero part; nothing when n <= 0 or mean == 0. */
+int pspin_shift_mean_couplings(double* d_c, long long n, double mean);
+/* apply_fmc_g2 (include/disorder.h:58): zero g2[i][j] and g2[j][i], i<j, when
+ * |omega_i - omega_j| > gamma; the diagonal is kept. */
+int pspin_apply_fmc_g2(double* d_g2, int N, const double* d_omega, double gamma);
+/* init_g4_mask (include/disorder.h:62): every quartet channel 0 (0x1). */
+int pspin_init_g4_mask(uint8_t* d_mask, int N);
+/* apply_fmc_g4 (include/disorder.h:66): overwrites the mask with the first
+ * channel whose balance holds, |((w_i+w_j)-w_k)-w_l| <= gamma (0x1), then
+ * |((w_i+w_l)-w_j)-w_k| (0x2), then |((w_i+w_k)-w_j)-w_l| (0x4), else 0,
+ * FP64 sums left to right: bit-exact against the reference's kernel. */
+int pspin_apply_fmc_g4(uint8_t* d_mask, int N, const double* d_omega, double gamma);
+/* count_active_g4 (include/disorder.h:69): set bits of the mask, counted on
+ * the GPU (the reference copies the mask to the host). */
+int pspin_count_active_g4(const uint8_t* d_mask, int N, long long* n_active);
+/* compute_energy (include/hamiltonian.h:11-17): FP64 H2 + H4 of one
+ * configuration, H4 = -sum_q sum_{ch in mask[q]} Re[g4_q f_ch(a)] with the
+ * channel products of src/hamiltonian.cu:101-108 (complex couplings
+ * allowed).  H2 sums over i<j, the convention of the reference's sweep and
+ * CPU oracle (src/mc.cu:164-179, src/test_delta_e.cu:38-43), not the i<=j of
+ * src/hamiltonian.cu:30-38.  d_g2 may be NULL (no two-body term).  The sums
+ * run in a fixed order, so the result is deterministic.  Allocates a
+ * workspace of one double per workgroup (at most 1025) per call, as the
+ * reference allocates its partial sums (src/hamiltonian.cu:168-196). */
+int pspin_compute_energy(const double* d_spins, const double* d_g2, const double* d_g4, const uint8_t* d_mask, int N,
+                         double* energy);
+/* init_spins_uniform (include/spins.h:9): 2N normals rescaled to
+ * sum |a|^2 = N.  init_spins_cube (include/spins_sparse.h:10): rescaled to
+ * sum |a|^4 = N. */
+int pspin_init_spins_uniform(double* d_spins, int N, unsigned long long seed);
+int pspin_init_spins_cube(double* d_spins, int N, unsigned long long seed);
+/* Colex decode of n quartet indices into sorted (i, j, k, l) int quadruples
+ * [n][4]: the device decoder every dense kernel uses (the round trip of
+ * src/test_quartet_index.cu:56-74). */
+int pspin_decode_quartets(const long long* d_q, int* d_ijkl, long long n);
+
 /* ---------------- parallel tempering (src/parallel_tempering.cu) -------- */
 /* Set up NT x nrep_per_temp replicas (nrep must equal their product),
  * betas from beta_sched[NT] (logical index t*nrep_per_temp+r, :273-278), the

@@ -7,6 +7,10 @@
 //       (include/mc.h:9-56)
 //   MCStateSparse + mc_sparse_* (include/mc_sparse.h:15-61)
 //   n_quartets / n_g4_total / n_pairs (include/disorder.h:32-44)
+//   generate_g2 / generate_g4 / rescale_* / shift_mean_couplings / apply_fmc_* /
+//       init_g4_mask / count_active_g4 (include/disorder.h:23-69),
+//       compute_energy (include/hamiltonian.h:11-17), init_spins_uniform
+//       (include/spins.h:9), init_spins_cube (include/spins_sparse.h:10)
 // Spins are exchanged as pspin_complex {x, y}, the layout of cuDoubleComplex.
 // Device buffers are owned by the opaque handle; the driver-side reads and
 // resets the reference does directly on d_accepted / d_proposed / d_quartets
@@ -162,4 +166,51 @@ inline void mc_compute_split_energies(const MCStateBase& s, double* e2, double* 
 }
 inline void mc_sparse_compute_split_energies(const MCStateSparse& s, double* e2, double* e4) {
     mc_compute_split_energies(s, e2, e4);
+}
+
+// ---------------------------------------------------------------------------
+// Free functions of include/disorder.h:23-69, include/hamiltonian.h:11-17,
+// include/spins.h:9 and include/spins_sparse.h:10 on device buffers of
+// pspin_complex (the layout of cuDoubleComplex).
+// ---------------------------------------------------------------------------
+inline void generate_g2(pspin_complex* d_g2, int N, double J, unsigned long long seed) {
+    pspin_die_on(pspin_generate_g2(reinterpret_cast<double*>(d_g2), N, J, seed), "generate_g2");
+}
+inline void generate_g4(pspin_complex* d_g4, int N, double J, unsigned long long seed) {
+    pspin_die_on(pspin_generate_g4(reinterpret_cast<double*>(d_g4), N, J, seed), "generate_g4");
+}
+inline void rescale_g2(pspin_complex* d_g2, int N, double J, long long n_surviving) {
+    pspin_die_on(pspin_rescale_g2(reinterpret_cast<double*>(d_g2), N, J, n_surviving), "rescale_g2");
+}
+inline void rescale_g4(pspin_complex* d_g4, int N, double J, long long n_surviving) {
+    pspin_die_on(pspin_rescale_g4(reinterpret_cast<double*>(d_g4), N, J, n_surviving), "rescale_g4");
+}
+inline void shift_mean_couplings(pspin_complex* d_c, long long n, double mean) {
+    pspin_die_on(pspin_shift_mean_couplings(reinterpret_cast<double*>(d_c), n, mean), "shift_mean_couplings");
+}
+inline void apply_fmc_g2(pspin_complex* d_g2, int N, const double* d_omega, double gamma) {
+    pspin_die_on(pspin_apply_fmc_g2(reinterpret_cast<double*>(d_g2), N, d_omega, gamma), "apply_fmc_g2");
+}
+inline void init_g4_mask(uint8_t* d_mask, int N) { pspin_die_on(pspin_init_g4_mask(d_mask, N), "init_g4_mask"); }
+inline void apply_fmc_g4(uint8_t* d_mask, int N, const double* d_omega, double gamma) {
+    pspin_die_on(pspin_apply_fmc_g4(d_mask, N, d_omega, gamma), "apply_fmc_g4");
+}
+inline long long count_active_g4(const uint8_t* d_mask, int N) {
+    long long n = 0;
+    pspin_die_on(pspin_count_active_g4(d_mask, N, &n), "count_active_g4");
+    return n;
+}
+inline double compute_energy(const pspin_complex* d_spins, const pspin_complex* d_g2, const pspin_complex* d_g4,
+                             const uint8_t* d_g4_mask, int N) {
+    double e = 0.0;
+    pspin_die_on(pspin_compute_energy(reinterpret_cast<const double*>(d_spins), reinterpret_cast<const double*>(d_g2),
+                                      reinterpret_cast<const double*>(d_g4), d_g4_mask, N, &e),
+                 "compute_energy");
+    return e;
+}
+inline void init_spins_uniform(pspin_complex* d_spins, int N, unsigned long long seed) {
+    pspin_die_on(pspin_init_spins_uniform(reinterpret_cast<double*>(d_spins), N, seed), "init_spins_uniform");
+}
+inline void init_spins_cube(pspin_complex* d_spins, int N, unsigned long long seed) {
+    pspin_die_on(pspin_init_spins_cube(reinterpret_cast<double*>(d_spins), N, seed), "init_spins_cube");
 }

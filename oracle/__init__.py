@@ -90,6 +90,9 @@ def lib() -> C.CDLL:
             "or_cpu_pt": (None, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _u8p, _dp, _u64p, C.c_int, C.c_long,
                                  C.c_int, _dp, _dp, _dp, _i64p, _i64p, _i64p, _i64p, C.POINTER(C.c_int), C.c_int,
                                  _dp]),
+            "or_cpu_pt_lists": (None, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _u8p, _dp, _u64p, C.c_int,
+                                       C.c_long, C.c_int, _dp, _dp, _dp, _i64p, _i64p, _i64p, _i64p,
+                                       C.POINTER(C.c_int), C.c_int, _dp]),
             "or_log_jn": (C.c_double, [C.c_int, C.c_double]),
             "or_lp_log_density": (C.c_double, [C.c_int, _dp, _dp, _dp]),
             "or_lp_gauge": (C.c_int, [C.c_int, _dp, _dp, _dp]),
@@ -304,16 +307,20 @@ def step_uniforms(rseed: int, sweep: int, step: int) -> np.ndarray:
     return u
 
 
-def cpu_pt(N, NT, nr, spins, g2, h2_active, g4, mask, beta_sched, rseeds, n_sweeps, seed48, measure_from, qbins=0):
+def cpu_pt(N, NT, nr, spins, g2, h2_active, g4, mask, beta_sched, rseeds, n_sweeps, seed48, measure_from, qbins=0,
+           lists=False):
     """The reference's PT loop on the CPU (FP64 sweeps, host exchange); returns per-slot sums
     (e, e2 of E/N over the slot's replicas; q2 and, with qbins > 0, a histogram of |q| over
-    [0, 1] for the slot's first two replicas)."""
+    [0, 1] for the slot's first two replicas).  lists=True visits each site's active quartets
+    from a table built once instead of the 3-type enumeration (same terms and per-term
+    arithmetic, another summation order)."""
     sp = _spins(spins).copy()
     out = dict(e=np.zeros(NT), e2=np.zeros(NT), q2=np.zeros(NT), acc=np.zeros(NT, np.int64),
                prop=np.zeros(NT, np.int64), ex_acc=np.zeros(NT - 1, np.int64), ex_prop=np.zeros(NT - 1, np.int64),
                qh=np.zeros((NT, max(qbins, 1))))
     n = C.c_int()
-    lib().or_cpu_pt(N, NT, nr, sp, np.ascontiguousarray(g2, dtype=np.float64).reshape(-1), int(h2_active),
+    fn = lib().or_cpu_pt_lists if lists else lib().or_cpu_pt
+    fn(N, NT, nr, sp, np.ascontiguousarray(g2, dtype=np.float64).reshape(-1), int(h2_active),
                     np.ascontiguousarray(g4, dtype=np.float64), np.ascontiguousarray(mask, dtype=np.uint8),
                     np.ascontiguousarray(beta_sched, dtype=np.float64), np.ascontiguousarray(rseeds, dtype=np.uint64),
                     n_sweeps, seed48, measure_from, out["e"], out["e2"], out["q2"], out["acc"], out["prop"],

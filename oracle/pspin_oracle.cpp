@@ -34,6 +34,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <array>
+#include <memory>
 #include <vector>
 #include <chrono>
 #ifdef _OPENMP
@@ -684,11 +686,116 @@ static double lcg48_next(uint64_t* x) {
     return (double)*x / 281474976710656.0;
 }
 
+// The same sweep with the quartet terms of each site listed once up front:
+// for the pair (i0, j0) it visits the active quartets holding i0, then those
+// holding j0 but not i0 -- the set the 3-type enumeration visits, minus the
+// inactive ones it skips -- with the same per-term FP64 arithmetic
+// (g (f_new - f_old)).  Only the order of the sum differs, so Delta E agrees
+// with ref_sweep_one to rounding.  It lets the statistical comparisons run at
+// N = 32 in seconds; ref_sweep_one stays the timed CPU baseline.
+struct SiteTerms {
+    std::vector<std::array<int, 4>> idx;
+    std::vector<int> ch;
+    std::vector<double> g;
+    std::vector<std::vector<int>> of_site;
+    SiteTerms(int N, const double* g4, const uint8_t* mask) : of_site(N) {
+        long long q = 0;
+        for (int l = 3; l < N; l++)
+            for (int k = 2; k < l; k++)
+                for (int j = 1; j < k; j++)
+                    for (int i = 0; i < j; i++, q++) {
+                        if (g4[q] == 0.0 || mask[q] == 0) continue;
+                        const int t = (int)g.size();
+                        idx.push_back({i, j, k, l});
+                        ch.push_back(mask_channel(mask[q]));
+                        g.push_back(g4[q]);
+                        for (int x : {i, j, k, l}) of_site[x].push_back(t);
+                    }
+    }
+};
+
+static void lists_sweep_one(int N, double* s, const double* g2, int h2_active, const SiteTerms& L, double beta,
+                            uint64_t rseed, uint64_t sweep, double* E, long long* acc) {
+    for (int step = 0; step < N / 2; step++) {
+        double u[6];
+        or_step_uniforms(rseed, sweep, step, u);
+        int i0, j0;
+        or_pair_pick(N, u[0], u[1], &i0, &j0);
+        double ai[2] = {s[2 * i0], s[2 * i0 + 1]}, aj[2] = {s[2 * j0], s[2 * j0 + 1]};
+        double ni[2], nj[2];
+        or_propose_sphere(ai, aj, u[2], u[3], u[4], ni, nj);
+        cplx di = {ni[0] - ai[0], ni[1] - ai[1]}, dj = {nj[0] - aj[0], nj[1] - aj[1]};
+        double dE = 0.0;
+        if (h2_active) {
+            cplx si = {0, 0}, sj = {0, 0};
+            for (int k = 0; k < N; k++) {
+                if (k == i0 || k == j0) continue;
+                cplx akc = cconj(ld(s, k));
+                si.re += g2[i0 * N + k] * akc.re; si.im += g2[i0 * N + k] * akc.im;
+                sj.re += g2[j0 * N + k] * akc.re; sj.im += g2[j0 * N + k] * akc.im;
+            }
+            dE -= cmul(di, si).re + cmul(dj, sj).re;
+            double gij = g2[i0 * N + j0];
+            cplx op = cmul({ai[0], ai[1]}, cconj({aj[0], aj[1]}));
+            cplx np = cmul({ni[0], ni[1]}, cconj({nj[0], nj[1]}));
+            dE -= gij * (np.re - op.re);
+        }
+        for (int pass = 0; pass < 2; pass++)
+            for (int t : L.of_site[pass == 0 ? i0 : j0]) {
+                const std::array<int, 4>& q4 = L.idx[t];
+                if (pass == 1 && (q4[0] == i0 || q4[1] == i0 || q4[2] == i0 || q4[3] == i0)) continue;
+                int idx[4] = {q4[0], q4[1], q4[2], q4[3]};
+                cplx fo = quartet_product(s, idx, L.ch[t]);
+                double sv[8];
+                for (int p = 0; p < 4; p++) {
+                    int x = idx[p];
+                    const double* src = (x == i0) ? ni : (x == j0) ? nj : &s[2 * x];
+                    sv[2 * p] = src[0]; sv[2 * p + 1] = src[1];
+                }
+                int loc[4] = {0, 1, 2, 3};
+                cplx fn = quartet_product(sv, loc, L.ch[t]);
+                dE -= L.g[t] * (fn.re - fo.re);
+            }
+        if (or_metropolis(beta, dE, u[5])) {
+            s[2 * i0] = ni[0]; s[2 * i0 + 1] = ni[1];
+            s[2 * j0] = nj[0]; s[2 * j0 + 1] = nj[1];
+            *E += dE;
+            (*acc)++;
+        }
+    }
+}
+
+static void cpu_pt_impl(bool lists, int N, int NT, int nr, double* spins, const double* g2, int h2_active,
+                        const double* g4, const uint8_t* mask, const double* beta_sched, const uint64_t* rseeds,
+                        int n_sweeps, long seed48, int measure_from, double* e_sum, double* e2_sum, double* q2_sum,
+                        long long* acc, long long* prop, long long* ex_acc, long long* ex_prop, int* n_meas, int qbins,
+                        double* qh_sum);
+
 void or_cpu_pt(int N, int NT, int nr, double* spins, const double* g2, int h2_active, const double* g4,
                const uint8_t* mask, const double* beta_sched, const uint64_t* rseeds, int n_sweeps, long seed48,
                int measure_from, double* e_sum, double* e2_sum, double* q2_sum, long long* acc, long long* prop,
                long long* ex_acc, long long* ex_prop, int* n_meas, int qbins, double* qh_sum) {
+    cpu_pt_impl(false, N, NT, nr, spins, g2, h2_active, g4, mask, beta_sched, rseeds, n_sweeps, seed48, measure_from,
+                e_sum, e2_sum, q2_sum, acc, prop, ex_acc, ex_prop, n_meas, qbins, qh_sum);
+}
+
+// or_cpu_pt with lists_sweep_one in place of ref_sweep_one.
+void or_cpu_pt_lists(int N, int NT, int nr, double* spins, const double* g2, int h2_active, const double* g4,
+                     const uint8_t* mask, const double* beta_sched, const uint64_t* rseeds, int n_sweeps, long seed48,
+                     int measure_from, double* e_sum, double* e2_sum, double* q2_sum, long long* acc, long long* prop,
+                     long long* ex_acc, long long* ex_prop, int* n_meas, int qbins, double* qh_sum) {
+    cpu_pt_impl(true, N, NT, nr, spins, g2, h2_active, g4, mask, beta_sched, rseeds, n_sweeps, seed48, measure_from,
+                e_sum, e2_sum, q2_sum, acc, prop, ex_acc, ex_prop, n_meas, qbins, qh_sum);
+}
+
+static void cpu_pt_impl(bool lists, int N, int NT, int nr, double* spins, const double* g2, int h2_active,
+                        const double* g4, const uint8_t* mask, const double* beta_sched, const uint64_t* rseeds,
+                        int n_sweeps, long seed48, int measure_from, double* e_sum, double* e2_sum, double* q2_sum,
+                        long long* acc, long long* prop, long long* ex_acc, long long* ex_prop, int* n_meas, int qbins,
+                        double* qh_sum) {
     const int R = NT * nr;
+    std::unique_ptr<SiteTerms> L;
+    if (lists) L.reset(new SiteTerms(N, g4, mask));
     std::vector<int> perm(R), slot(R);
     for (int i = 0; i < R; i++) perm[i] = i;
     std::vector<double> E(R);
@@ -705,8 +812,12 @@ void or_cpu_pt(int N, int NT, int nr, double* spins, const double* g2, int h2_ac
         for (int r = 0; r < R; r++) {
             long long a = 0;
             const int t = slot[r];
-            ref_sweep_one(N, spins + (size_t)2 * N * r, g2, h2_active, g4, mask, beta_sched[t], rseeds[r],
-                          (uint64_t)sw, &E[r], &a);
+            if (lists)
+                lists_sweep_one(N, spins + (size_t)2 * N * r, g2, h2_active, *L, beta_sched[t], rseeds[r],
+                                (uint64_t)sw, &E[r], &a);
+            else
+                ref_sweep_one(N, spins + (size_t)2 * N * r, g2, h2_active, g4, mask, beta_sched[t], rseeds[r],
+                              (uint64_t)sw, &E[r], &a);
             if (sw >= measure_from) {
                 acc[t] += a;
                 prop[t] += N / 2;

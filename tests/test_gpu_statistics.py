@@ -13,6 +13,14 @@ specific heat Cv = N beta^2 (<e^2> - <e>^2) (src/analysis_pt.cpp:846-900),
 conj(a_k^2) / N between the two replicas of a slot, the overlap of
 notes/quantities/observables.tex); the Metropolis acceptance and the
 exchange rate per temperature pair.
+
+Systems: N=10 fully connected (the pair-row kernel; per-replica and
+ladder-shared pair streams) and N=32 uniform FMC gamma=0.05 (the group
+kernel that carries the FMC and comb configurations) with FP32 and with
+BF16 hi/lo couplings.  At N=32 the CPU chains visit each site's active
+quartets from a table (oracle.cpu_pt(lists=True): the same terms and
+arithmetic as the 3-type enumeration, pinned against it in
+tests/test_oracle_physics.py).
 """
 import concurrent.futures as cf
 
@@ -21,15 +29,30 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-N, NT, NR = 10, 5, 2
+NT, NR = 5, 2
 BURN, MEAS = 500, 2000
 QB = 4  # |q| histogram bins
 
 
-def _gpu_run(pspin, seed, beta, groups):
+# name: (N, fmc mode, gamma, sweep kernel, coupling format, pair groups)
+SYSTEMS = {
+    "fc10": (10, 0, 0.0, 0, 0, 0),
+    "fc10_ladder_pairs": (10, 0, 0.0, 0, 0, NR),
+    "fmc32_groups_f32": (32, 2, 0.05, 3, 0, NR),
+    "fmc32_groups_bf16": (32, 2, 0.05, 3, 1, NR),
+}
+
+
+def _gpu_run(pspin, seed, beta, system):
+    N, fmc, gamma, kernel, fmt, groups = SYSTEMS[system]
     beta = np.asarray(beta)
-    cfg = pspin.SimConfig.default(N=N, nrep=NT * NR, seed=seed, fmc_mode=0)
+    cfg = pspin.SimConfig.default(N=N, nrep=NT * NR, seed=seed, fmc_mode=fmc, gamma=gamma)
     st = pspin.PSpin(cfg, pspin.DENSE)
+    if fmt:
+        st.set_coupling_format(fmt)
+    if kernel:
+        st.set_kernel(kernel, 0)
+        assert st.info.kernel == kernel
     st.set_pair_schedule(groups)
     st.pt_init(NT, NR, beta, seed + 5000)
     s0 = st.spins()
@@ -60,14 +83,15 @@ def _gpu_run(pspin, seed, beta, groups):
     st.close()
     cv = N * beta ** 2 * (e2 / n - (e / n) ** 2)
     obs = np.concatenate([e / n, cv, q2 / n, (qh / n).reshape(-1), [a.sum() / p.sum()], xa / xp])
-    return obs, dict(s0=s0, g2=g2, g4=g4, mask=mask, rs=rs)
+    return obs, dict(s0=s0, g2=g2, g4=g4, mask=mask, rs=rs, h2=st.info.h2_active)
 
 
-def _cpu_run(oracle, seed, beta, d):
+def _cpu_run(oracle, seed, beta, d, system):
+    N = SYSTEMS[system][0]
     beta = np.asarray(beta)
     rs = d["rs"] ^ np.uint64(0x9E3779B97F4A7C15)
-    out = oracle.cpu_pt(N, NT, NR, d["s0"], d["g2"], 1, d["g4"], d["mask"], beta, rs, BURN + MEAS,
-                        seed + 5000 + 17, BURN, qbins=QB)
+    out = oracle.cpu_pt(N, NT, NR, d["s0"], d["g2"], d["h2"], d["g4"], d["mask"], beta, rs, BURN + MEAS,
+                        seed + 5000 + 17, BURN, qbins=QB, lists=N > 16)
     n = out["n"]
     e = out["e"] / (n * NR)
     cv = N * beta ** 2 * (out["e2"] / (n * NR) - e ** 2)
@@ -76,19 +100,21 @@ def _cpu_run(oracle, seed, beta, d):
     return np.concatenate([e, cv, q2, (out["qh"] / n).reshape(-1), [acc], out["ex_acc"] / out["ex_prop"]])
 
 
-@pytest.mark.parametrize("groups", [0, NR])
-def test_gpu_pt_matches_reference_logic_statistically(pspin, oracle, groups):
-    """groups = NR: each ladder's replicas share one pair stream (the layout
-    the benchmark uses), the ladders keep independent streams."""
+@pytest.mark.parametrize("system", list(SYSTEMS))
+def test_gpu_pt_matches_reference_logic_statistically(pspin, oracle, system):
+    """Pair groups = NR: each ladder's replicas share one pair stream (the
+    layout the benchmark uses), the ladders keep independent streams."""
+    print(f"[statistics] {system}", flush=True)
+    N = SYSTEMS[system][0]
     T, beta = oracle.temperatures(NT, 0.5, 2.0, 1)
     seeds = [1000 + 7 * s for s in range(12)]
     gpu, data = [], []
     for sd in seeds:
-        o, d = _gpu_run(pspin, sd, beta, groups)
+        o, d = _gpu_run(pspin, sd, beta, system)
         gpu.append(o)
         data.append(d)
-    with cf.ThreadPoolExecutor(max_workers=8) as ex:
-        cpu = list(ex.map(lambda a: _cpu_run(oracle, a[0], beta, a[1]), zip(seeds, data)))
+    with cf.ThreadPoolExecutor(max_workers=12) as ex:
+        cpu = list(ex.map(lambda a: _cpu_run(oracle, a[0], beta, a[1], system), zip(seeds, data)))
     gpu, cpu = np.array(gpu), np.array(cpu)
     diff = gpu - cpu
     S = len(seeds)

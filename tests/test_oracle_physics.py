@@ -288,3 +288,33 @@ def test_cpu_pt_overlap_histogram_brackets_q2(oracle):
     hi = (np.arange(1, Q + 1) / Q) ** 2
     assert np.all(qh @ lo <= q2 + 1e-9) and np.all(q2 <= qh @ hi + 1e-9), (qh @ lo, q2, qh @ hi)
     assert np.all(qh[:, 1:].sum(axis=1) > 0)  # the histogram is not degenerate
+
+
+@pytest.mark.parametrize("N,fmc,gamma", [(9, 0, 0.0), (12, 2, 0.1), (11, 1, 1.0)])
+def test_cpu_pt_site_lists_follow_the_enumeration(oracle, N, fmc, gamma):
+    """The list sweep visits the same active quartets as the reference's
+    3-type enumeration with the same per-term arithmetic, so on the same
+    inputs the two CPU PT loops take the same decisions: identical counters
+    and spins and energies equal to rounding.  A list that drops the
+    quartets holding both sites, or counts them twice, changes Delta E by
+    O(1) and the chains part within a few sweeps."""
+    rng = np.random.default_rng(N)
+    NT, nr = 3, 2
+    w = oracle.omega(N, fmc, 5) if fmc else np.zeros(N)
+    mask = oracle.g4_mask(N, fmc, w, gamma)
+    g4 = rng.normal(size=len(mask))
+    g4[rng.random(len(mask)) < 0.05] = 0.0  # zero couplings are skipped by both
+    g2 = rng.normal(size=(N, N))
+    g2 = (g2 + g2.T) / 2
+    a0 = rng.normal(size=(NT * nr, 2 * N))
+    a0 *= math.sqrt(N) / np.linalg.norm(a0, axis=1, keepdims=True)
+    beta = np.array([0.3, 0.9, 2.5])
+    seeds = oracle.replica_seeds(77, NT * nr)
+    outs = [oracle.cpu_pt(N, NT, nr, a0.reshape(-1), g2, 1, g4, mask, beta, seeds, 60, 9, 10, lists=lst)
+            for lst in (False, True)]
+    a, b = outs
+    for k in ("acc", "prop", "ex_acc", "ex_prop"):
+        np.testing.assert_array_equal(a[k], b[k])
+    np.testing.assert_allclose(a["spins"], b["spins"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(a["e"], b["e"], rtol=1e-12)
+    assert a["acc"].sum() > 0
