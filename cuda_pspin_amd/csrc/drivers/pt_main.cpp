@@ -292,11 +292,22 @@ int main(int argc, char** argv) {
     }
     printf("\n");
     if (FILE* ft = fopen(path("time.txt").c_str(), "w")) {
+#ifdef PSPIN_PT_SPARSE
+        // the sparse driver adds the number of sampled quartets
+        // (src/parallel_tempering_sparse.cu:505-517)
+        fprintf(ft,
+                "# N NT nrep_per_temp total_replicas mc_iterations pt_freq save_freq Tmax Tmin n_sparse total_time_s "
+                "ms_per_sweep\n");
+        fprintf(ft, "%d %d %d %d %d %d %d %.6f %.6f %d %.6f %.6f\n", cfg.N, NT, nrep_per_temp, total,
+                cfg.mc_iterations, pt_freq, cfg.save_freq, Tmax, Tmin, st.n_quartets, elapsed,
+                elapsed / cfg.mc_iterations * 1e3);
+#else
         fprintf(ft,
                 "# N NT nrep_per_temp total_replicas mc_iterations pt_freq save_freq Tmax Tmin total_time_s "
                 "ms_per_sweep\n");
         fprintf(ft, "%d %d %d %d %d %d %d %.6f %.6f %.6f %.6f\n", cfg.N, NT, nrep_per_temp, total, cfg.mc_iterations,
                 pt_freq, cfg.save_freq, Tmax, Tmin, elapsed, elapsed / cfg.mc_iterations * 1e3);
+#endif
         fclose(ft);
     }
     fclose(fout);
