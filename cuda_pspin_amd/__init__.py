@@ -132,6 +132,7 @@ def load_library() -> C.CDLL:
         "pspin_apply_fmc_g4": (C.c_int, [P, C.c_int, P, C.c_double]),
         "pspin_count_active_g4": (C.c_int, [P, C.c_int, C.POINTER(C.c_longlong)]),
         "pspin_compute_energy": (C.c_int, [P, P, P, P, C.c_int, C.POINTER(C.c_double)]),
+        "pspin_compute_energy_noalloc": (C.c_int, [P, P, P, P, C.c_int, C.POINTER(C.c_double)]),
         "pspin_init_spins_uniform": (C.c_int, [P, C.c_int, C.c_ulonglong]),
         "pspin_init_spins_cube": (C.c_int, [P, C.c_int, C.c_ulonglong]),
         "pspin_decode_quartets": (C.c_int, [P, P, C.c_longlong]),

@@ -8,6 +8,8 @@
 // threads in a fixed LDS tree to one partial.  One last workgroup adds the
 // two-body term (i<j) and the partials in index order, so the result does
 // not depend on scheduling.
+#include <mutex>
+
 #include "pspin_internal.h"
 
 namespace ps {
@@ -99,6 +101,35 @@ hipError_t launch_energy_one(const double* spins, const double* g2, const double
     return hipGetLastError();
 }
 
+// Workspace of pspin_compute_energy_noalloc: the H4 partials and the
+// result, one copy per GPU for the life of the process.  Host code must not
+// pass `scratch` itself to a kernel (the name is a host-side stand-in); it
+// uses the device address that hipGetSymbolAddress returns for the current
+// device, kept in scratch_dev.
+constexpr int kScratchDoubles = kEnMaxParts + 1;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static double* scratch_dev[kMaxDevices];
+
+// The current device's address of `scratch`, looked up once per device.
+hipError_t scratch_address(double** out) {
+    int d = 0;
+    hipError_t e = hipGetDevice(&d);
+    if (e != hipSuccess) return e;
+    if (d < 0 || d >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[d]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_dev[d] = (double*)p;
+    }
+    *out = scratch_dev[d];
+    return hipSuccess;
+}
+
 }  // namespace ps
 
 extern "C" {
@@ -117,6 +148,28 @@ int pspin_compute_energy(const double* d_spins, const double* d_g2, const double
     if (ws) (void)hipFree(ws);
     if (e != hipSuccess) {
         ps::set_error(std::string("HIP error ") + hipGetErrorString(e) + " in pspin_compute_energy");
+        return PSPIN_ERR_HIP;
+    }
+    return PSPIN_OK;
+}
+
+int pspin_compute_energy_noalloc(const double* d_spins, const double* d_g2, const double* d_g4, const uint8_t* d_mask,
+                                 int N, double* energy) {
+    if (!(d_spins && d_g4 && d_mask && energy && N >= 4)) {
+        ps::set_error("pspin_compute_energy_noalloc: bad argument (device spins, g4, mask and a host result)");
+        return PSPIN_ERR_ARG;
+    }
+    if (N > ps::kMaxN || ps::energy_parts(N) + 1 > ps::kScratchDoubles) {
+        ps::set_error("pspin_compute_energy_noalloc: N = " + std::to_string(N) + " is above the maximum of " +
+                      std::to_string(ps::kMaxN) + " sites");
+        return PSPIN_ERR_ARG;
+    }
+    double* ws = nullptr;
+    hipError_t e = ps::scratch_address(&ws);
+    if (e == hipSuccess) e = ps::launch_energy_one(d_spins, d_g2, d_g4, d_mask, N, ws);
+    if (e == hipSuccess) e = hipMemcpy(energy, ws + ps::energy_parts(N), sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        ps::set_error(std::string("HIP error ") + hipGetErrorString(e) + " in pspin_compute_energy_noalloc");
         return PSPIN_ERR_HIP;
     }
     return PSPIN_OK;

@@ -187,6 +187,17 @@ int pspin_count_active_g4(const uint8_t* d_mask, int N, long long* n_active);
  * reference allocates its partial sums (src/hamiltonian.cu:168-196). */
 int pspin_compute_energy(const double* d_spins, const double* d_g2, const double* d_g4, const uint8_t* d_mask, int N,
                          double* energy);
+/* pspin_compute_energy without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its workspace (1025 doubles: the H4 partials and the result) is a static
+ * device array of the library, one per GPU, looked up once per device; the
+ * call makes no hipMalloc, hipFree or other allocating call.  Limit: N <= 256
+ * (PSPIN_ERR_ARG above).  Calls on one GPU share the array, so calls that
+ * overlap on one GPU (from several host threads) race; every partial is
+ * rewritten by each call, so nothing needs clearing between calls. */
+int pspin_compute_energy_noalloc(const double* d_spins, const double* d_g2, const double* d_g4, const uint8_t* d_mask,
+                                 int N, double* energy);
 /* init_spins_uniform (include/spins.h:9): 2N normals rescaled to
  * sum |a|^2 = N.  init_spins_cube (include/spins_sparse.h:10): rescaled to
  * sum |a|^4 = N. */
