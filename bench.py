@@ -122,7 +122,7 @@ def learned_vs_plain(args):
     lp.collect(args.lp_snapshots, 1, 1)
     t_collect = time.perf_counter() - t0
     t0 = time.perf_counter()
-    hist = lp.train(args.lp_train_steps, batch=4096, lag=1, lr=1e-3)
+    hist = lp.train(args.lp_train_steps, batch=4096, lag=args.lp_lag, lr=1e-3)
     t_train = time.perf_counter() - t0
     lp.set_scale(args.lp_scale)
     lp.set_mean_scale(args.lp_scale ** 2)
@@ -173,7 +173,8 @@ def learned_vs_plain(args):
     out["config"] = {"workload": "dense spherical PT N=64 fmc uniform gamma=0.05, NT=32 x nrep=8, T in "
                                  f"[{args.Tmin},{args.Tmax}] geometric, exchange every sweep",
                      "learned": f"one move per {args.lp_every} sweeps, scale {args.lp_scale}, mean scale "
-                                f"{args.lp_scale ** 2:.3g}", "blocks": args.lp_blocks, "sweeps_per_block": S,
+                                f"{args.lp_scale ** 2:.3g}, training lag {args.lp_lag}", "blocks": args.lp_blocks,
+                     "sweeps_per_block": S,
                      "burn_in_sweeps": args.lp_burn,
                      "sweep_kernel": {"kernel": tuned[0], "block": tuned[1], "trial_ms_per_sweep": tuned[2]}}
     return out
@@ -497,6 +498,8 @@ def main():
     ap.add_argument("--lp-train-steps", type=int, default=1500)
     ap.add_argument("--lp-scale", type=float, default=0.2)
     ap.add_argument("--lp-every", type=int, default=8)
+    ap.add_argument("--lp-lag", type=int, default=1,
+                    help="training pairs (snapshot t, replica r) -> (t + lag, r): the move learns to predict lag sweeps ahead")
     ap.add_argument("--lp-blocks", type=int, default=4)
     ap.add_argument("--lp-block", type=int, default=16384)
     args = ap.parse_args()

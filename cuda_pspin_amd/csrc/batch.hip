@@ -58,8 +58,8 @@ __device__ __forceinline__ BatchDraw batch_draw(uint64_t seed, uint64_t sweep, i
     d.ij = (ri < rj ? ri : rj) | ((ri < rj ? rj : ri) << 16);
     d.pad = 0;
     const double u = u32_to_unit(x.z);
-    sincos(2.0 * M_PI * u32_to_unit(x.w), &d.s1, &d.c1);
-    sincos(2.0 * M_PI * u32_to_unit(y.x), &d.s2, &d.c2);
+    sincospi(2.0 * u32_to_unit(x.w), &d.s1, &d.c1);  // 2 pi u, u in (0, 1): no large-argument reduction
+    sincospi(2.0 * u32_to_unit(y.x), &d.s2, &d.c2);
     d.q1 = sqrt(1.0 - u);
     d.q2 = sqrt(u);
     d.lu = -log(u32_to_unit(y.y));

@@ -137,14 +137,15 @@ __device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t gkey, uint6
     p.i0 = ri < rj ? ri : rj;
     p.j0 = ri < rj ? rj : ri;
     double ua = u32_to_unit(x.z), up1 = u32_to_unit(x.w), up2 = u32_to_unit(y.x), uacc = u32_to_unit(y.y);
+    // angles 2 pi u with u in (0, 1): sincospi needs no large-argument reduction
     if (CUBE) {
         p.c[0] = sqrt(1.0 - ua);
         p.c[1] = sqrt(ua);
     } else {
-        sincos(2.0 * M_PI * ua, &p.c[1], &p.c[0]);
+        sincospi(2.0 * ua, &p.c[1], &p.c[0]);
     }
-    sincos(2.0 * M_PI * up1, &p.c[3], &p.c[2]);
-    sincos(2.0 * M_PI * up2, &p.c[5], &p.c[4]);
+    sincospi(2.0 * up1, &p.c[3], &p.c[2]);
+    sincospi(2.0 * up2, &p.c[5], &p.c[4]);
     // u < exp(-beta dE)  <=>  dE < -log(u)/beta  (src/mc.cu:289-296)
     p.thr = (beta == 0.0) ? INFINITY : -log(uacc) / beta;
     return p;

@@ -57,6 +57,39 @@ def test_batch_matches_single_states(pspin, oracle, pt_freq):
     b.close()
 
 
+def test_batch_matches_single_states_at_cfg3_shape(pspin, oracle):
+    """The benchmark's shape per sample (24 temperatures in [0.2, 1.6] x 4
+    replicas = 96 chains, uniform FMC gamma 0.05, N sampled quartets) at its
+    two largest sizes, N = 48 and 64, with an exchange after every sweep and
+    the FP64 resync at sweep 8."""
+    Ns, seeds = [48, 64, 64, 48], [42, 43, 44, 45]
+    NT, nr = 24, 4
+    T, beta = oracle.temperatures(NT, 0.2, 1.6, 1)
+    base = pspin.SimConfig.default(fmc_mode=2, gamma=0.05)
+    b = pspin.PSpinBatch(base, Ns, seeds, NT, nr, beta)
+    b.set_resync_interval(8)
+    sweeps = 12
+    b.run(sweeps, 1)
+    for s, (N, seed) in enumerate(zip(Ns, seeds)):
+        print(f"[batch cfg3 shape] N={N} seed={seed}", flush=True)
+        st = _single(pspin, N, seed, NT, nr, beta, 2, 0.05)
+        st.set_resync_interval(8)
+        st.pt_run(sweeps, 1)
+        np.testing.assert_array_equal(b.perm(s), st.pt_perm())
+        _, _, acc_s, prop_s = st.pt_exchange_stats()
+        acc_b, prop_b = b.exchange_stats(s)
+        np.testing.assert_array_equal(acc_b, acc_s)
+        np.testing.assert_array_equal(prop_b, prop_s)
+        Eb, ab, pb = b.results(s)
+        Es, as_, ps_ = st.results()
+        np.testing.assert_allclose(b.spins(s), st.spins(), atol=1e-9)
+        np.testing.assert_allclose(Eb, Es, rtol=0, atol=1e-5 * N)
+        np.testing.assert_array_equal(ab, as_)
+        np.testing.assert_array_equal(pb, ps_)
+        st.close()
+    b.close()
+
+
 @pytest.mark.parametrize("NT,nr", [(4, 24), (4, 32), (5, 32), (2, 96)])
 def test_batch_workgroup_shapes_match_single_states(pspin, oracle, NT, nr):
     """96, 128, 160 and 192 chains per sample: 8 lanes per chain in 768 and

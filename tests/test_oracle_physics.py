@@ -318,3 +318,27 @@ def test_cpu_pt_site_lists_follow_the_enumeration(oracle, N, fmc, gamma):
     np.testing.assert_allclose(a["spins"], b["spins"], rtol=0, atol=1e-12)
     np.testing.assert_allclose(a["e"], b["e"], rtol=1e-12)
     assert a["acc"].sum() > 0
+
+
+def test_sparse_select_hand_trace(oracle):
+    """A selection worked out by hand (src/mc_sparse.cu:340-349,489-537).
+    N = 5: quartets q = 0..4 are (0,1,2,3), (0,1,2,4), (0,1,3,4), (0,2,3,4),
+    (1,2,3,4).  mask = [0x1, 0, 0x6, 0x4, 0x2], unit couplings
+    [0.5, 1.0, -2.0, 0.0, 3.0]: q1 has no channel, q3 a zero coupling, q2 two
+    channels, so the candidates q*3+ch in ascending order are [0, 7, 8, 13].
+    The seed is 2^64 - 4000, so the shuffle's splitmix64 state seed + 4000
+    wraps to 0 and its draws are the published SplitMix64 sequence for seed 0:
+    r0 = 0xE220A8397B1DCDAF, r1 = 0x6E789E6AA1B965F4.
+      i = 0: j = 0 + r0 mod 4 = 3 -> swap positions 0, 3: [13, 7, 8, 0]
+      i = 1: j = 1 + r1 mod 3 = 1 -> no swap
+    so the two selected terms are 13 = (q4, ch 1) and 7 = (q2, ch 1), with
+    g = g_unit * J4 sqrt(N / 2) + J4_0 N / 2."""
+    assert 0xE220A8397B1DCDAF % 4 == 3 and 0x6E789E6AA1B965F4 % 3 == 0
+    mask = np.array([1, 0, 6, 4, 2], dtype=np.uint8)
+    g_unit = np.array([0.5, 1.0, -2.0, 0.0, 3.0])
+    J4, J40 = 1.25, 0.3
+    q, g, nnz = oracle.sparse_select(5, mask, g_unit, 2, 2 ** 64 - 4000, J4, J40)
+    assert nnz == 4
+    assert q.tolist() == [[1, 2, 3, 4, 1], [0, 1, 3, 4, 1]]
+    s, m = J4 * math.sqrt(5 / 2), J40 * 5 / 2
+    assert g.tolist() == [3.0 * s + m, -2.0 * s + m]
