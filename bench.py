@@ -113,6 +113,8 @@ def learned_vs_plain(args):
     st = P.PSpin(P.SimConfig.default(N=N, nrep=R, seed=args.seed, fmc_mode=2, gamma=0.05, dev=0), P.DENSE)
     if args.no_graphs:
         st.set_profiling(True)  # per-launch events instead of captured graphs
+    if args.direct_launches:
+        st.set_launch_mode(P.LAUNCH_DIRECT)  # PT rounds as ordinary launches (same chains)
     st.pt_init(NT, nr, 1.0 / T, args.seed + 5000)
     st.pt_run(args.lp_burn, 1)
     tuned = st.autotune(8)  # the sweep kernel both runs use; restores the state
@@ -490,6 +492,9 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, help="only run the counter-calibration copy of this size")
     ap.add_argument("--workload", choices=["dense", "batch", "learned", "cfg1", "cfg4"], default="dense")
     ap.add_argument("--no-graphs", action="store_true", help="learned workload: launch kernels one by one")
+    ap.add_argument("--direct-launches", action="store_true",
+                    help="learned workload: PT rounds as ordinary launches, not cooperative ones (identical chains; "
+                         "for profiler runs, which end cleanly only without cooperative launches)")
     ap.add_argument("--batch-chunk", type=int, default=0,
                     help="batch workload: sweeps per launch (0: autotuned); a fixed value gives every launch the "
                          "same size, for counter runs")

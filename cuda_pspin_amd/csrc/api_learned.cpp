@@ -224,15 +224,18 @@ int ensure_graph(pspin_learned* lp, int pt_freq, int every) {
 }
 
 // FP64 Hamiltonian terms of the state: H4 quartets (colex walk of the dense
-// coupling table and mask) and, when H2 is active, the pairs i < j.
+// coupling table and mask) and, when H2 is active, the pairs i < j.  A term
+// is {i | j << 7 | k << 14 | l << 21 | ch << 28, 0, g} (ch 3: pair), and
+// the terms of one (j, k, l, ch) are consecutive: the energy kernel sums
+// g a_i over such a run before multiplying by the product of the others.
 int build_terms(pspin_learned* lp) {
     pspin_state* st = lp->st;
     const int N = lp->N;
     std::vector<uint4> terms;
-    auto push = [&](uint32_t idx, uint32_t ch, double g) {
+    auto push = [&](uint32_t i, uint32_t j, uint32_t k, uint32_t l, uint32_t ch, double g) {
         uint64_t b;
         std::memcpy(&b, &g, 8);
-        terms.push_back(make_uint4(idx, ch, (uint32_t)b, (uint32_t)(b >> 32)));
+        terms.push_back(make_uint4(i | j << 7 | k << 14 | l << 21 | ch << 28, 0u, (uint32_t)b, (uint32_t)(b >> 32)));
     };
     const long long nq = ps::n_quartets(N);
     std::vector<double> g4((size_t)nq);
@@ -240,32 +243,34 @@ int build_terms(pspin_learned* lp) {
     LP_TRY_HIP(hipMemcpyAsync(g4.data(), st->d_g4_64, (size_t)nq * 8, hipMemcpyDeviceToHost, st->stream));
     LP_TRY_HIP(hipMemcpyAsync(mask.data(), st->d_mask, (size_t)nq, hipMemcpyDeviceToHost, st->stream));
     LP_TRY_HIP(hipStreamSynchronize(st->stream));
-    long long q = 0;
+    long long q = 0;  // colex index of (0, j, k, l)
     for (int l = 3; l < N; l++)
         for (int k = 2; k < l; k++)
-            for (int j = 1; j < k; j++)
-                for (int i = 0; i < j; i++, q++) {
-                    const uint8_t m = mask[(size_t)q];
-                    const double g = g4[(size_t)q];
-                    if (m == 0 || g == 0.0) continue;
-                    const uint32_t ch = (m & 1) ? 0 : ((m & 2) ? 1 : 2);
-                    push((uint32_t)i | (uint32_t)j << 8 | (uint32_t)k << 16 | (uint32_t)l << 24, ch, g);
-                }
+            for (int j = 1; j < k; q += j, j++)
+                for (uint32_t ch = 0; ch < 3; ch++)
+                    for (int i = 0; i < j; i++) {
+                        const uint8_t m = mask[(size_t)(q + i)];
+                        const double g = g4[(size_t)(q + i)];
+                        if (m == 0 || g == 0.0 || ch != ((m & 1) ? 0u : ((m & 2) ? 1u : 2u))) continue;
+                        push(i, j, k, l, ch, g);
+                    }
+    if (q != nq) {
+        ps::set_error("learned move: quartet walk does not cover the coupling table");
+        return PSPIN_ERR_ARG;
+    }
     if (st->h2_active) {
         std::vector<double> g2((size_t)N * N);
         LP_TRY_HIP(hipMemcpyAsync(g2.data(), st->d_g2_64, (size_t)N * N * 8, hipMemcpyDeviceToHost, st->stream));
         LP_TRY_HIP(hipStreamSynchronize(st->stream));
-        for (int i = 0; i < N; i++)
-            for (int j = i + 1; j < N; j++)
-                if (g2[(size_t)i * N + j] != 0.0) push((uint32_t)i | (uint32_t)j << 8, 3, g2[(size_t)i * N + j]);
+        for (int j = 1; j < N; j++)
+            for (int i = 0; i < j; i++)
+                if (g2[(size_t)i * N + j] != 0.0) push(i, j, 0, 0, 3, g2[(size_t)i * N + j]);
     }
     lp->n_terms = (long long)terms.size();
     LP_TRY_HIP(dmalloc(&lp->d_terms, terms.size()));
     if (!terms.empty())
         LP_TRY_HIP(hipMemcpy(lp->d_terms, terms.data(), terms.size() * sizeof(uint4), hipMemcpyHostToDevice));
-    // each split is one workgroup's worth of terms (every thread one chunk)
-    const long long per = ps::lp_energy_terms_per_split();
-    lp->n_split = (int)std::max(1LL, (lp->n_terms + per - 1) / per);
+    lp->n_split = ps::lp_energy_splits(lp->n_terms);
     LP_TRY_HIP(ps::lp_energy_prepare());
     LP_TRY_HIP(dmalloc(&lp->d_part, (size_t)lp->n_split * 2 * lp->R));
     return PSPIN_OK;

@@ -104,11 +104,16 @@ __device__ double log_jn(int n, double alpha) {
     return fs + log(h * wsum(sum));
 }
 
+// coordinates per lane (d = lane + 64 t) at the largest N; loops run over
+// all kMaxPer slots, unrolled, so the per-lane arrays stay in registers
+constexpr int kMaxPer = (2 * kLearnedMaxN + 63) / 64;
+
 // log density of unit vector y (this lane's coordinates d = lane + 64 t)
 // under the projected Gaussian with means mu and log-scales ls.
-__device__ double log_proj_density(int dim, const double* y, const double* mu, const double* ls, int per) {
+__device__ __forceinline__ double log_proj_density(int dim, const double* y, const double* mu, const double* ls) {
     double A = 0.0, B = 0.0, C = 0.0, L = 0.0;
-    for (int t = 0; t < per; t++) {
+#pragma unroll
+    for (int t = 0; t < kMaxPer; t++) {
         const double iv = exp(-2.0 * ls[t]);
         A += y[t] * y[t] * iv;
         B += y[t] * mu[t] * iv;
@@ -123,7 +128,6 @@ __device__ double log_proj_density(int dim, const double* y, const double* mu, c
     return -hd * log(2.0 * M_PI) - L - 0.5 * C - hd * log(A) + log_jn(dim - 1, B / sqrt(A));
 }
 
-constexpr int kMaxPer = (2 * kMaxN + 63) / 64;  // coordinates per lane
 
 
 // Gauge-fix amplitudes a [N][2] with c and write the FP64 frame coordinates
@@ -158,14 +162,15 @@ __device__ __forceinline__ double clampd(double x, double lo, double hi) { retur
 
 __global__ __launch_bounds__(64) void k_lp_propose(MoveArgs a) {
     const int r = blockIdx.x, lane = threadIdx.x;
-    const int N = a.N, dim = 2 * N, per = (dim + 63) / 64;
+    const int N = a.N, dim = 2 * N;
     const uint64_t seed = a.seeds[r], mv = a.mctr[r];
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const double* U = a.U + (size_t)dim * r;
     const float* O = a.Of + (size_t)a.out_ld * r;
     double mu[kMaxPer], ls[kMaxPer], z[kMaxPer];
     double z2 = 0.0;
-    for (int t = 0; t < per; t++) {
+#pragma unroll
+    for (int t = 0; t < kMaxPer; t++) {
         const int d = lane + 64 * t;
         mu[t] = 0.0;
         ls[t] = 0.0;
@@ -186,16 +191,19 @@ __global__ __launch_bounds__(64) void k_lp_propose(MoveArgs a) {
     }
     z2 = wsum(z2);
     const double inv = 1.0 / sqrt(z2);
-    for (int t = 0; t < per; t++) z[t] *= inv;  // unit vector y (zero beyond dim)
-    for (int t = 0; t < per; t++)
+#pragma unroll
+    for (int t = 0; t < kMaxPer; t++) z[t] *= inv;  // unit vector y (zero beyond dim)
+#pragma unroll
+    for (int t = 0; t < kMaxPer; t++)
         if (lane + 64 * t >= dim) mu[t] = 0.0, ls[t] = 0.0;
-    const double lqf = log_proj_density(dim, z, mu, ls, per);
+    const double lqf = log_proj_density(dim, z, mu, ls);
     // back to the physical frame: a' = sqrt(N) y conj(c); (re, im) of a
     // site sit in neighbouring lanes
     const double cr = a.gc[2 * r], ci = a.gc[2 * r + 1], rn = sqrt((double)N);
     double* P = a.prop + (size_t)dim * r;
     double pa[kMaxPer];
-    for (int t = 0; t < per; t++) {
+#pragma unroll
+    for (int t = 0; t < kMaxPer; t++) {
         const double y = z[t] * rn;
         const double other = __shfl_xor(y, 1, 64);
         const int d = lane + 64 * t;
@@ -207,7 +215,8 @@ __global__ __launch_bounds__(64) void k_lp_propose(MoveArgs a) {
     // gauge of the proposal, from registers
     double best = -1.0;
     int bk = 0x7FFFFFFF;
-    for (int t = 0; t < per; t++) {
+#pragma unroll
+    for (int t = 0; t < kMaxPer; t++) {
         const double o = __shfl_xor(pa[t], 1, 64);
         const int d = lane + 64 * t;
         if (d < dim && !(d & 1)) {
@@ -229,14 +238,16 @@ __global__ __launch_bounds__(64) void k_lp_propose(MoveArgs a) {
     }
     const int ts = (2 * bk) >> 6, ls_ = (2 * bk) & 63;
     double sel = 0.0;
-    for (int t = 0; t < per; t++)
+#pragma unroll
+    for (int t = 0; t < kMaxPer; t++)
         if (t == ts) sel = pa[t];
     const double kre = __shfl(sel, ls_, 64), kim = __shfl(sel, ls_ + 1, 64);
     const double rk = sqrt(best);
     const double cpr = kre / rk, cpi = -kim / rk;
     double* Up = a.Up + (size_t)dim * r;
     uint16_t* Xp = a.X0p + (size_t)a.in_pad * r;
-    for (int t = 0; t < per; t++) {
+#pragma unroll
+    for (int t = 0; t < kMaxPer; t++) {
         const double o = __shfl_xor(pa[t], 1, 64);
         const int d = lane + 64 * t;
         const double re = (d & 1) ? o : pa[t];
@@ -255,114 +266,112 @@ __global__ __launch_bounds__(64) void k_lp_propose(MoveArgs a) {
     }
 }
 
-struct c64d {
+// 16-byte aligned, so LDS reads and writes of one are single ds_read_b128 /
+// ds_write_b128 (a wave's 64 consecutive entries then touch every bank
+// equally; two 8-byte halves conflict two-way)
+struct alignas(16) c64d {
     double x, y;
 };
 __device__ __forceinline__ c64d cmul(c64d p, c64d q) { return {p.x * q.x - p.y * q.y, p.x * q.y + p.y * q.x}; }
 __device__ __forceinline__ c64d ccj(c64d p) { return {p.x, -p.y}; }
 
 // Full FP64 energies of 2R configurations (current states, then proposals).
-// A workgroup takes kEnCfg configurations (amplitudes in LDS as
-// [config][site]) and a range of the term list; each thread walks kEnChunk
-// consecutive terms for all kEnCfg configurations.  The list is in colex
-// order (l, k, j outer to inner), so a thread keeps the products of a_k, a_l
-// and of a_j with them across its terms and a quartet term costs one
-// amplitude read and two FMAs per configuration:
-//   ch0  a_i a_j a_k* a_l* = a_i [a_j (a_k a_l)*]
-//   ch1  a_i a_j* a_k* a_l = a_i [a_j* (a_k* a_l)]
-//   ch2  a_i a_j* a_k a_l* = a_i [a_j* (a_k a_l*)]
-// Pair terms (ch 3) read both amplitudes.  The term list is split over
-// gridDim.y; the accept kernel combines the partial sums in fixed order.
-constexpr int kEnCfg = 4;
-constexpr int kEnChunk = 16;
-constexpr int kEnThreads = 256;
+// A workgroup holds 64 configurations, one per lane, with their amplitudes
+// in LDS as [site][configuration] (16 B per entry: a wave's read of one site
+// is 1 KB of consecutive addresses, free of bank conflicts), and takes one
+// of n_split ranges of the term list; its 16 waves take a sixteenth of the
+// range each.  A wave loads its terms 64 at a time with one vector load and
+// broadcasts each with readlane, so a term's indices are scalars and its
+// branches uniform.
+//
+// A term is {x, 0, g}: x = i | j << 7 | k << 14 | l << 21 | ch << 28.  The
+// list is grouped by key = x >> 7 = (j, k, l, ch): a group's terms differ
+// only in i, and its energy is -Re[(sum_i g a_i) u] with u fixed by the key:
+//   ch0  a_i a_j a_k* a_l*   u = a_j (a_k a_l)*
+//   ch1  a_i a_j* a_k* a_l   u = a_j* (a_k* a_l)
+//   ch2  a_i a_j* a_k a_l*   u = a_j* (a_k a_l*)
+//   ch3  a_i a_j*  (pairs)   u = a_j*
+// so a term costs one amplitude read and two FMAs per configuration, and a
+// group three reads and two complex products more.  The waves' sums are
+// added in wave order, and the accept kernel adds the n_split partials in a
+// fixed order.
+constexpr int kEnLanes = 64;
+constexpr int kEnWaves = 16;
+constexpr int kEnThreads = kEnLanes * kEnWaves;
+constexpr int kEnMaxSplit = 64;
+constexpr int kEnTermsPerSplit = 2048;
 
-__global__ __launch_bounds__(kEnThreads) void k_lp_energy(const double* __restrict__ cur,
-                                                          const double* __restrict__ prop, int R, int N,
-                                                          const uint4* __restrict__ terms, long long nt,
-                                                          double* __restrict__ part) {
-    __shared__ c64d sl[kEnCfg][kLearnedMaxN + 1];  // [config][site]
-    __shared__ double red[kEnThreads / 64][kEnCfg];
-    const int c0 = blockIdx.x * kEnCfg;
-    for (int e = threadIdx.x; e < kEnCfg * N; e += kEnThreads) {
-        const int q = e / N, k = e - q * N, c = c0 + q;
+// A run of terms: the open group's key, its sum S = sum_i g a_i and its u.
+struct EnRun {
+    c64d S, u;
+    uint32_t key;
+};
+
+// Close the open group into en and open `key` when it differs.
+__device__ __forceinline__ void en_group(EnRun& r, uint32_t key, const c64d* A, double& en) {
+    if (key == r.key) return;
+    en -= r.S.x * r.u.x - r.S.y * r.u.y;
+    r.S = {0.0, 0.0};
+    const uint32_t ch = key >> 21;
+    const c64d aj = A[(key & 127u) * kEnLanes];
+    if (ch == 3u) {
+        r.u = ccj(aj);
+    } else {
+        const c64d ak = A[((key >> 7) & 127u) * kEnLanes], al = A[((key >> 14) & 127u) * kEnLanes];
+        if (ch == 0u) r.u = cmul(aj, ccj(cmul(ak, al)));
+        else if (ch == 1u) r.u = cmul(ccj(aj), cmul(ccj(ak), al));
+        else r.u = cmul(ccj(aj), cmul(ak, ccj(al)));
+    }
+    r.key = key;
+}
+
+__global__ __launch_bounds__(kEnThreads, 8) void k_lp_energy(const double* __restrict__ cur,
+                                                             const double* __restrict__ prop, int R, int N,
+                                                             const uint4* __restrict__ terms, long long nt,
+                                                             double* __restrict__ part) {
+    extern __shared__ c64d amp[];  // [N][kEnLanes]
+    __shared__ double red[kEnWaves][kEnLanes];
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c0 = blockIdx.x * kEnLanes;
+    for (int e = threadIdx.x; e < N * kEnLanes; e += kEnThreads) {
+        const int k = e >> 6, c = c0 + (e & 63);
         c64d v = {0.0, 0.0};
         if (c < 2 * R) {
             const double* src = c < R ? cur + (size_t)2 * N * c : prop + (size_t)2 * N * (c - R);
-            v = {src[2 * k], src[2 * k + 1]};
+            const double2 d = *(const double2*)(src + 2 * k);  // rows of 2N doubles: 16-byte aligned
+            v = {d.x, d.y};
         }
-        sl[q][k] = v;
+        amp[e] = v;
     }
     __syncthreads();
-    const long long per_block = (long long)kEnThreads * kEnChunk;
-    const long long t0 = ((long long)blockIdx.y * kEnThreads + threadIdx.x) * kEnChunk;
-    const long long t1 = min(nt, t0 + kEnChunk);
-    (void)per_block;
-    double en[kEnCfg];
-    c64d qkl[kEnCfg], rkl[kEnCfg], skl[kEnCfg];
-    double p0x[kEnCfg], p0y[kEnCfg], p1x[kEnCfg], p1y[kEnCfg], p2x[kEnCfg], p2y[kEnCfg];
-#pragma unroll
-    for (int q = 0; q < kEnCfg; q++) en[q] = 0.0;
-    uint32_t ckl = 0xFFFFFFFFu, cj = 0xFFFFFFFFu;
-    // the next term is read while this one is used (one memory latency per chunk, not per term)
-    uint4 wn = t0 < t1 ? terms[t0] : make_uint4(0u, 0u, 0u, 0u);
-    for (long long t = t0; t < t1; t++) {
-        const uint4 w = wn;
-        if (t + 1 < t1) wn = terms[t + 1];
-        const double g = __hiloint2double((int)w.w, (int)w.z);
-        const uint32_t i = w.x & 255u, j = (w.x >> 8) & 255u;
-        if (w.y == 3u) {
-#pragma unroll
-            for (int q = 0; q < kEnCfg; q++) {
-                const c64d ai = sl[q][i], aj = sl[q][j];
-                en[q] -= g * (ai.x * aj.x + ai.y * aj.y);
-            }
-            continue;
-        }
-        const uint32_t kl = w.x >> 16;
-        if (kl != ckl) {
-#pragma unroll
-            for (int q = 0; q < kEnCfg; q++) {
-                const c64d ak = sl[q][kl & 255u], al = sl[q][kl >> 8];
-                qkl[q] = ccj(cmul(ak, al));
-                rkl[q] = cmul(ccj(ak), al);
-                skl[q] = cmul(ak, ccj(al));
-            }
-            ckl = kl;
-            cj = 0xFFFFFFFFu;
-        }
-        if (j != cj) {
-#pragma unroll
-            for (int q = 0; q < kEnCfg; q++) {
-                const c64d aj = sl[q][j];
-                const c64d u0 = cmul(aj, qkl[q]), u1 = cmul(ccj(aj), rkl[q]), u2 = cmul(ccj(aj), skl[q]);
-                p0x[q] = u0.x; p0y[q] = u0.y;
-                p1x[q] = u1.x; p1y[q] = u1.y;
-                p2x[q] = u2.x; p2y[q] = u2.y;
-            }
-            cj = j;
-        }
-        const bool c0sel = w.y == 0u, c1sel = w.y == 1u;
-#pragma unroll
-        for (int q = 0; q < kEnCfg; q++) {
-            const double px = c0sel ? p0x[q] : (c1sel ? p1x[q] : p2x[q]);
-            const double py = c0sel ? p0y[q] : (c1sel ? p1y[q] : p2y[q]);
-            const c64d ai = sl[q][i];
-            en[q] -= g * (ai.x * px - ai.y * py);
+    const long long nq = (long long)gridDim.y * kEnWaves;
+    const long long wq = (long long)blockIdx.y * kEnWaves + wv;
+    const long long t0 = nt * wq / nq, t1 = nt * (wq + 1) / nq;
+    const c64d* A = amp + lane;
+    double en = 0.0;
+    EnRun run{{0.0, 0.0}, {0.0, 0.0}, 0xFFFFFFFFu};
+    for (long long b = t0; b < t1; b += kEnLanes) {
+        // one load per 64 terms (other waves cover its latency)
+        const uint4 w = b + lane < t1 ? terms[b + lane] : make_uint4(0u, 0u, 0u, 0u);
+        const int n = (int)min((long long)kEnLanes, t1 - b);
+        for (int e = 0; e < n; e++) {
+            const uint32_t x = __builtin_amdgcn_readlane(w.x, e);
+            const double g = __hiloint2double(__builtin_amdgcn_readlane((int)w.w, e),
+                                              __builtin_amdgcn_readlane((int)w.z, e));
+            const c64d ai = A[(x & 127u) * kEnLanes];
+            en_group(run, x >> 7, A, en);
+            run.S.x += g * ai.x;
+            run.S.y += g * ai.y;
         }
     }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < kEnCfg; q++) {
-        const double v = wsum(en[q]);
-        if (lane == 0) red[wv][q] = v;
-    }
+    en -= run.S.x * run.u.x - run.S.y * run.u.y;
+    red[wv][lane] = en;
     __syncthreads();
-    if (threadIdx.x < kEnCfg) {
+    if (threadIdx.x < kEnLanes) {
         const int c = c0 + threadIdx.x;
         if (c < 2 * R) {
             double v = 0.0;
-            for (int x = 0; x < kEnThreads / 64; x++) v += red[x][threadIdx.x];
+            for (int x = 0; x < kEnWaves; x++) v += red[x][threadIdx.x];
             part[(size_t)blockIdx.y * 2 * R + c] = v;
         }
     }
@@ -370,7 +379,7 @@ __global__ __launch_bounds__(kEnThreads) void k_lp_energy(const double* __restri
 
 __global__ __launch_bounds__(64) void k_lp_accept(MoveArgs a) {
     const int r = blockIdx.x, lane = threadIdx.x;
-    const int N = a.N, dim = 2 * N, per = (dim + 63) / 64;
+    const int N = a.N, dim = 2 * N;
     double ec = 0.0, en = 0.0;
     for (int s = lane; s < a.n_split; s += 64) {
         ec += a.part[(size_t)s * 2 * a.R + r];
@@ -387,7 +396,8 @@ __global__ __launch_bounds__(64) void k_lp_accept(MoveArgs a) {
     double v[kMaxPer], mu[kMaxPer], ls[kMaxPer];
     double v2 = 0.0, lw = 0.0, lwp = 0.0;
     const double* P = a.prop + (size_t)dim * r;
-    for (int t = 0; t < per; t++) {
+#pragma unroll
+    for (int t = 0; t < kMaxPer; t++) {
         const int d = lane + 64 * t;
         v[t] = mu[t] = ls[t] = 0.0;
         if (d < dim) {
@@ -407,8 +417,9 @@ __global__ __launch_bounds__(64) void k_lp_accept(MoveArgs a) {
     lw = wsum(lw);
     lwp = wsum(lwp);
     const double inv = 1.0 / sqrt(v2);
-    for (int t = 0; t < per; t++) v[t] *= inv;
-    const double lqr = log_proj_density(dim, v, mu, ls, per);
+#pragma unroll
+    for (int t = 0; t < kMaxPer; t++) v[t] *= inv;
+    const double lqr = log_proj_density(dim, v, mu, ls);
     const double lqf = a.dbg[8 * r + 0];
     const double beta = a.betas[r];
     const double logr = -beta * (en - ec) + (lwp - lw) + (lqr - lqf);
@@ -447,10 +458,17 @@ __global__ __launch_bounds__(64) void k_lp_accept(MoveArgs a) {
     }
 }
 
-hipError_t lp_energy_prepare() { return hipSuccess; }
+hipError_t lp_energy_prepare() {
+    // dynamic LDS of [N][64] amplitudes: 128 KB at the largest N
+    return hipFuncSetAttribute((const void*)k_lp_energy, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(kLearnedMaxN * kEnLanes * sizeof(c64d)));
+}
 
-int lp_energy_blocks(int R) { return (2 * R + kEnCfg - 1) / kEnCfg; }
-long long lp_energy_terms_per_split() { return (long long)kEnThreads * kEnChunk; }
+int lp_energy_blocks(int R) { return (2 * R + kEnLanes - 1) / kEnLanes; }
+int lp_energy_splits(long long n_terms) {
+    // about kEnTermsPerSplit terms per workgroup (128 per wave), at most kEnMaxSplit ranges
+    return (int)std::max(1LL, std::min<long long>(kEnMaxSplit, (n_terms + kEnTermsPerSplit - 1) / kEnTermsPerSplit));
+}
 
 hipError_t launch_lp_prep(const MoveArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_lp_prep, dim3(a.R), dim3(64), 0, s, a);
@@ -463,8 +481,8 @@ hipError_t launch_lp_propose(const MoveArgs& a, hipStream_t s) {
 hipError_t launch_lp_energy(const MoveArgs& a, const uint4* terms, long long nt, double* part, hipStream_t s) {
     if (a.N > kLearnedMaxN) return hipErrorInvalidValue;
     dim3 grid(lp_energy_blocks(a.R), a.n_split);
-    hipLaunchKernelGGL(k_lp_energy, grid, dim3(kEnThreads), 0, s, a.spins, (const double*)a.prop, a.R, a.N, terms,
-                       nt, part);
+    hipLaunchKernelGGL(k_lp_energy, grid, dim3(kEnThreads), (size_t)a.N * kEnLanes * sizeof(c64d), s, a.spins,
+                       (const double*)a.prop, a.R, a.N, terms, nt, part);
     return hipGetLastError();
 }
 hipError_t launch_lp_accept(const MoveArgs& a, hipStream_t s) {

@@ -73,6 +73,29 @@ __device__ __forceinline__ void put(const uint4& v0, const uint4& v1, uint16_t (
     }
 }
 
+// Epilogue of a 32 x 32 accumulator block at rows mb.., column n of the lane:
+// accumulator element r of lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
+template <int EPI>
+__device__ __forceinline__ void epilogue(const GemmArgs& a, const f32x16& acc, float* Cf, int mb, int n, int lane) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int m = mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const float v = acc[r];
+        if (EPI == kEpiF32) {
+            Cf[(long long)m * a.ldc + n] = v;
+        } else if (EPI == kEpiBiasF32) {
+            a.Cf[(long long)m * a.ldc + n] = v + a.bias[n];
+        } else if (EPI == kEpiBiasSilu) {
+            const float z = v + a.bias[n];
+            a.Cf[(long long)m * a.ldc + n] = z;
+            a.Cb[(long long)m * a.ldcb + n] = f2bf(silu(z));
+        } else {
+            const float z = a.Z[(long long)m * a.ldz + n];
+            a.Cb[(long long)m * a.ldcb + n] = f2bf(v * dsilu(z));
+        }
+    }
+}
+
 // Block tile 64 x 64, four waves of 32 x 32, K steps of 64 with the next
 // step's operands fetched into registers while the current step's MFMAs run
 // (double-buffered LDS, one barrier per step).  gridDim.z > 1 splits K:
@@ -126,26 +149,69 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs a) {
         cur ^= 1;
     }
     if (rows) a.rowsum[(long long)blockIdx.z * a.M + m0 + threadIdx.x] = rs;
-    float* Cf = a.Cf + (long long)blockIdx.z * a.M * a.ldc;
-    // accumulator element r of lane l: row 8(r/4) + 4(l/32) + r%4, column l%32
-    const int n = n0 + wn + fr;
+    epilogue<EPI>(a, acc, a.Cf + (long long)blockIdx.z * a.M * a.ldc, m0 + wm, n0 + wn + fr, lane);
+}
+
+// Forward product of a few rows (the learned move's M = R rows): one wave
+// per 32 x 32 output tile, so M = 256 rows and N = 512 give 128 workgroups
+// instead of 32.  Both operands are k-contiguous (X [M][K], W [N][K]) and are
+// loaded straight from global memory into the MFMA's operand layout (lane l
+// takes row l % 32 and the 8 elements at k + 8 (l / 32) of each 16-wide k
+// step): no LDS, no barrier.  K = 64 KS is fixed at compile time, so the
+// loads of up to four 64-wide k blocks are in flight at once (all of them
+// for K <= 256) and the wave waits for memory about once, not once a block.
+template <int EPI, int KS>
+__global__ __launch_bounds__(64) void k_gemm_rows(GemmArgs a) {
+    constexpr int D = KS < 4 ? KS : 4;  // k blocks in flight
+    const int lane = threadIdx.x;
+    const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+    const int fr = lane & 31, fk = (lane >> 5) * 8;
+    const uint16_t* Ap = a.A + (long long)(m0 + fr) * a.a_m + fk;
+    const uint16_t* Bp = a.B + (long long)(n0 + fr) * a.b_n + fk;
+    bf16x8 fa[D][4], fb[D][4];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int m = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const float v = acc[r];
-        if (EPI == kEpiF32) {
-            Cf[(long long)m * a.ldc + n] = v;
-        } else if (EPI == kEpiBiasF32) {
-            a.Cf[(long long)m * a.ldc + n] = v + a.bias[n];
-        } else if (EPI == kEpiBiasSilu) {
-            const float z = v + a.bias[n];
-            a.Cf[(long long)m * a.ldc + n] = z;
-            a.Cb[(long long)m * a.ldcb + n] = f2bf(silu(z));
-        } else {
-            const float z = a.Z[(long long)m * a.ldz + n];
-            a.Cb[(long long)m * a.ldcb + n] = f2bf(v * dsilu(z));
+    for (int p = 0; p < D; p++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            fa[p][u] = *(const bf16x8*)(Ap + 64 * p + 16 * u);
+            fb[p][u] = *(const bf16x8*)(Bp + 64 * p + 16 * u);
+        }
+    // keep the loads above issued before any MFMA (the scheduler would
+    // otherwise interleave them with the MFMAs, a few loads in flight)
+    __builtin_amdgcn_sched_barrier(0);
+    f32x16 acc = {};
+#pragma unroll
+    for (int kb = 0; kb < KS; kb++) {
+        const int p = kb % D;
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[p][u], fb[p][u], acc, 0, 0, 0);
+        if (kb + D < KS) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                fa[p][u] = *(const bf16x8*)(Ap + 64 * (kb + D) + 16 * u);
+                fb[p][u] = *(const bf16x8*)(Bp + 64 * (kb + D) + 16 * u);
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
+    epilogue<EPI>(a, acc, a.Cf, m0, n0 + fr, lane);
+}
+
+template <int EPI>
+static hipError_t launch_rows(const GemmArgs& g, hipStream_t s) {
+    dim3 grid(g.N / 32, g.M / 32), block(64);
+    switch (g.K / 64) {
+    case 1: hipLaunchKernelGGL((k_gemm_rows<EPI, 1>), grid, block, 0, s, g); break;
+    case 2: hipLaunchKernelGGL((k_gemm_rows<EPI, 2>), grid, block, 0, s, g); break;
+    case 3: hipLaunchKernelGGL((k_gemm_rows<EPI, 3>), grid, block, 0, s, g); break;
+    case 4: hipLaunchKernelGGL((k_gemm_rows<EPI, 4>), grid, block, 0, s, g); break;
+    case 5: hipLaunchKernelGGL((k_gemm_rows<EPI, 5>), grid, block, 0, s, g); break;
+    case 6: hipLaunchKernelGGL((k_gemm_rows<EPI, 6>), grid, block, 0, s, g); break;
+    case 7: hipLaunchKernelGGL((k_gemm_rows<EPI, 7>), grid, block, 0, s, g); break;
+    case 8: hipLaunchKernelGGL((k_gemm_rows<EPI, 8>), grid, block, 0, s, g); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 template <bool AKC, bool BKC>
@@ -161,8 +227,17 @@ static hipError_t launch_epi(const GemmArgs& g, int epi, int splits, hipStream_t
     return hipGetLastError();
 }
 
+// Rows at or below which a forward product takes k_gemm_rows, and its
+// largest K.
+constexpr int kRowsKernelMaxM = 512;
+constexpr int kRowsKernelMaxK = 512;
+
 hipError_t launch_gemm(const GemmArgs& g, int epi, int splits, hipStream_t s) {
     const bool akc = g.a_k == 1, bkc = g.b_k == 1;
+    if (akc && bkc && splits == 1 && g.M <= kRowsKernelMaxM && (epi == kEpiBiasSilu || epi == kEpiBiasF32) &&
+        g.M > 0 && g.M % 32 == 0 && g.N > 0 && g.N % 32 == 0 && g.K > 0 && g.K % 64 == 0 &&
+        g.K <= kRowsKernelMaxK && g.a_m % 8 == 0 && g.b_n % 8 == 0)
+        return epi == kEpiBiasSilu ? launch_rows<kEpiBiasSilu>(g, s) : launch_rows<kEpiBiasF32>(g, s);
     if ((akc ? g.a_m : g.a_k) % 8 || (bkc ? g.b_n : g.b_k) % 8) return hipErrorInvalidValue;
     if (!akc && g.a_m != 1) return hipErrorInvalidValue;
     if (!bkc && g.b_n != 1) return hipErrorInvalidValue;

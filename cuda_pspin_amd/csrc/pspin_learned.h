@@ -80,8 +80,8 @@ hipError_t launch_lp_prep(const MoveArgs& a, hipStream_t s);
 hipError_t launch_lp_propose(const MoveArgs& a, hipStream_t s);
 hipError_t launch_lp_energy(const MoveArgs& a, const uint4* terms, long long nt, double* part, hipStream_t s);
 hipError_t lp_energy_prepare();  // once per process and device, before the first launch
-int lp_energy_blocks(int R);     // workgroups per term split
-long long lp_energy_terms_per_split();
+int lp_energy_blocks(int R);     // workgroups per term split (64 configurations each)
+int lp_energy_splits(long long n_terms);  // term ranges (partial sums per configuration)
 constexpr int kLearnedMaxN = 128;
 hipError_t launch_lp_accept(const MoveArgs& a, hipStream_t s);
 hipError_t launch_lp_record(const double* spins, const double* betas, int R, int dim, float* out, float* obeta,

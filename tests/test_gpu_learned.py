@@ -94,8 +94,9 @@ def _slot_betas(st, beta_sched):
     return b
 
 
-def test_learned_move_replays_on_oracle(pspin, oracle):
-    st = _state(pspin, N=16, nrep=64, seed=11)
+@pytest.mark.parametrize("fmc", [0, 2])  # 2: uniform FMC, quartets in channels 0 and 1
+def test_learned_move_replays_on_oracle(pspin, oracle, fmc):
+    st = _state(pspin, N=16, nrep=64, seed=11, fmc=fmc)
     N, R = st.N, st.nrep
     T, beta = oracle.temperatures(8, 0.4, 2.0, 1)
     st.pt_init(8, 8, beta, 99)
