@@ -13,10 +13,12 @@
 // workgroup of a sweep launch): slots are staged in LDS once, lane r carries
 // ladder r's moving energy and replica down the levels in registers (level t
 // only swaps slots t and t+1, so slot t is final after level t and the values
-// of slot t+1 before it are the untouched ones), every lane computes the draw its rank would
-// take, X_{lane+1} = A_{lane+1} X + C_{lane+1}, lanes fetch their draw by rank
-// with one permute, and the state advances to the draw of lane (count - 1)
-// by a lane read: no LDS round trip on the chain from one level to the next.
+// of slot t+1 before it are the untouched ones).  Before the levels, the lanes
+// compute in parallel every draw the round can take (X_{k+1} = A_{k+1} X + C_{k+1})
+// and store logf of each; a level reads its draws by rank and compares their
+// logs with delta, so the chain from one level to the next holds one LDS
+// read and a few FP64 operations.  A draw within 3e-5 of the decision
+// boundary in log space is decided by the FP64 test on the draw itself.
 // More ladders (k_pt_exchange_wide) walk column blocks of 64 in LDS.
 #include "exchange_dev.h"
 

@@ -49,18 +49,22 @@ __device__ __forceinline__ void dev_store(T* p, T v) {
 
 // LDS of the one-wave exchange (n = NT nrep slots): sE[n] energies by slot,
 // sEr[n] energies by replica (as loaded), sB[NT] betas, sacc[NT] accepted
-// swaps per level, sctr[4 NT] the exchange counters as loaded, sX[n] the
-// lcg48 states of every draw the round can take, sX0 the 48-bit state and
-// sser the series counter as loaded, sP[n] replicas by slot.
+// swaps per level, sctr[4 NT] the exchange counters as loaded, sX0 the
+// 48-bit state and sser the series counter as loaded, sAB the accept bits of
+// each lane (64 levels per word, 64 words per group of 64 levels), sLU[n]
+// logf of every draw the round can take, sP[n + 1] replicas by slot (the
+// last entry takes the writes of lanes past the ladders).
+__host__ __device__ inline int exchange_ab_words(int NT) { return (NT - 1 + 63) / 64 * 64; }
 struct ExLds {
     double* sE;
     double* sEr;
     double* sB;
     long long* sacc;
     long long* sctr;
-    uint64_t* sX;
     uint64_t* sX0;
     long long* sser;
+    unsigned long long* sAB;
+    float* sLU;
     int* sP;
     __device__ ExLds(void* lds, int NT, int n) {
         sE = (double*)lds;
@@ -68,10 +72,11 @@ struct ExLds {
         sB = sEr + n;
         sacc = (long long*)(sB + NT);
         sctr = sacc + NT;
-        sX = (uint64_t*)(sctr + 4 * NT);
-        sX0 = sX + n;
+        sX0 = (uint64_t*)(sctr + 4 * NT);
         sser = (long long*)(sX0 + 1);
-        sP = (int*)(sser + 1);
+        sAB = (unsigned long long*)(sser + 1);
+        sLU = (float*)(sAB + exchange_ab_words(NT));
+        sP = (int*)(sLU + n);
     }
 };
 
@@ -119,9 +124,9 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
 }
 
 // The level chain: lane r carries ladder r's moving energy and replica; a
-// level's draws are read by rank from the precomputed states, so the chain
-// holds no permute, no 64-bit product and no global access.  The result goes
-// out after the last level.
+// level's draws are read by rank from the precomputed logs, so the chain
+// holds no permute, no 64-bit product, no exponential and no global access.
+// The result goes out after the last level.
 __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
     // An opaque copy of the lane index: in a kernel that runs the exchange
     // between sweeps, nothing lane-derived here may be hoisted out of that loop
@@ -132,14 +137,14 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
     ExLds L(lds, NT, n);
     const uint64_t X0 = *L.sX0;
     {
-        // X_{k+1} for k = 0 .. nr (NT - 1) - 1: lane l jumps l + 1 steps, then
-        // 64 steps at a time (lane 63's coefficients).
+        // draw k (k = 0 .. nr (NT - 1) - 1) is X_{k+1}: lane l jumps l + 1
+        // steps, then 64 steps at a time (lane 63's coefficients)
         uint64_t A, C;
         lcg48_coeffs((uint64_t)lane + 1, &A, &C);
         const uint64_t A64 = readlane64(A, 63), C64 = readlane64(C, 63);
         uint64_t x = (A * X0 + C) & kMask48;
         for (int k = lane; k < nr * (NT - 1); k += 64) {
-            L.sX[k] = x;
+            L.sLU[k] = logf((float)lcg48_unit(x));
             x = (A64 * x + C64) & kMask48;
         }
     }
@@ -147,39 +152,74 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    int pc = valid ? L.sP[lane] : 0;         // replica carried down the column
-    double ec = valid ? L.sE[lane] : 0.0;    // its energy
+    // The level loop is kept free of divergent branches and of stores that
+    // wait on a lane mask (each costs the lone wave issue cycles on every
+    // level): lanes past the ladders read ladder nr - 1's slots, unused, and
+    // write their slot to the spare entry sP[n]; every lane reads the draw
+    // log of its rank; accepts are kept as bits and counted after the loop.
+    const int lr = valid ? lane : nr - 1;
+    int pc = L.sP[lr];          // replica carried down the column
+    double ec = L.sE[lr];       // its energy
     double bt = L.sB[0];
-    int pn = valid ? L.sP[nr + lane] : 0;    // slot t+1 before level t
-    double en = valid ? L.sE[nr + lane] : 0.0;
+    int pn = L.sP[nr + lr];     // slot t+1 before level t
+    double en = L.sE[nr + lr];
     double bn = L.sB[1];
     int base = 0;  // draws taken so far
+    unsigned long long abits = 0;  // this lane's accepts, bit t % 64
     for (int t = 0; t < NT - 1; t++) {
         const int pb = pn;
         const double eb = en, b1 = bn;
-        if (t + 2 < NT) {  // next level's untouched slot, off the chain
-            pn = valid ? L.sP[(t + 2) * nr + lane] : 0;
-            en = valid ? L.sE[(t + 2) * nr + lane] : 0.0;
-            bn = L.sB[t + 2];
-        }
+        const int t2 = t + 2 < NT ? t + 2 : NT - 1;  // next level's untouched slot, off the chain
+        pn = L.sP[t2 * nr + lr];
+        en = L.sE[t2 * nr + lr];
+        bn = L.sB[t2];
         const double delta = (bt - b1) * (ec - eb);
         const bool need = valid && !(delta >= 0.0);
         const unsigned long long m = __ballot(need);
-        bool accept = valid;
-        if (need) accept = accept_below(lcg48_unit(L.sX[base + __popcll(m & lt)]), delta);
+        // rank k of this lane's draw (k < n: base + popcount stays within the
+        // nr (NT - 1) draws of the round)
+        const int k = base + __popcll(m & lt);
+        // u < exp(delta) decided from lu = logf((float)u), |lu - log u| < 1e-5
+        // (u rounded to FP32, then logf within a few ulp of |lu| <= 34; u = 0
+        // gives -inf): outside a 3e-5 band around delta the order of lu and
+        // delta is that of u and exp(delta).  Below delta = -700 every u > 0
+        // (at least 2^-48) exceeds exp(delta); that case goes to FP64 too.
+        const double dd = delta - (double)L.sLU[k];
+        // (bitwise: no short-circuit branches)
+        bool accept = valid & (!need | (dd > 3e-5));
+        const bool band = need & ((delta < -700.0) | !((dd > 3e-5) | (dd < -3e-5)));
+        if (__ballot(band)) {  // about 6e-5 of draws: the decision in FP64 on the draw itself
+            if (band) {
+                uint64_t A, C;
+                lcg48_coeffs((uint64_t)k + 1, &A, &C);
+                accept = lcg48_unit((A * X0 + C) & kMask48) < exp_band(delta);
+            }
+        }
         base += __popcll(m);
         // slot t keeps the carried replica unless the pair swaps (slot t is not
         // read again: the prefetch is two slots ahead)
-        if (valid) L.sP[t * nr + lane] = accept ? pb : pc;
-        if (!accept) {  // otherwise the carried replica moves down to slot t+1
-            pc = pb;
-            ec = eb;
-        }
+        L.sP[valid ? t * nr + lane : n] = accept ? pb : pc;
+        // a swap moves the carried replica down to slot t+1; otherwise slot
+        // t+1's replica is carried on
+        pc = accept ? pc : pb;
+        ec = accept ? ec : eb;
         bt = b1;
-        const int nacc = __popcll(__ballot(valid && accept));
-        if (lane == 0) L.sacc[t] = nacc;
+        abits |= (unsigned long long)accept << (t & 63);
+        if ((t & 63) == 63 || t == NT - 2) {  // uniform: every 64 levels and at the end
+            L.sAB[(t >> 6) * 64 + lane] = abits;
+            abits = 0;
+        }
     }
     if (valid) L.sP[(NT - 1) * nr + lane] = pc;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int t = lane; t < NT - 1; t += 64) {  // accepted swaps per level
+        const unsigned long long* w = L.sAB + (t >> 6) * 64;
+        long long c = 0;
+        for (int r = 0; r < nr; r++) c += (w[r] >> (t & 63)) & 1ull;
+        L.sacc[t] = c;
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -201,7 +241,9 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
         if (valid) dev_store(a.series + (size_t)(round % a.series_cap) * nr + lane, ec);
     }
     if (lane == 0) {
-        dev_store(a.rng48, base > 0 ? L.sX[base - 1] : X0);
+        uint64_t A, C;  // the state after the round's base draws
+        lcg48_coeffs((uint64_t)base, &A, &C);
+        dev_store(a.rng48, (A * X0 + C) & kMask48);
         if (a.series) dev_store(a.series_n, *L.sser + 1);
     }
 }
