@@ -111,6 +111,18 @@ __device__ __forceinline__ float wave_sum(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// The pair of a step from two uniform words (src/mc.cu:129-141).
+__device__ __forceinline__ void pick_pair(uint32_t a, uint32_t b, int N, int& i0, int& j0) {
+    const double u1 = u32_to_unit(a), u2 = u32_to_unit(b);
+    int ri = (int)(u1 * N);
+    if (ri >= N) ri = N - 1;
+    int rj = (int)(u2 * (N - 1));
+    if (rj >= N - 1) rj = N - 2;
+    if (rj >= ri) rj++;
+    i0 = ri < rj ? ri : rj;
+    j0 = ri < rj ? rj : ri;
+}
+
 // Draws and trigonometry for one step (pair pick src/mc.cu:129-141; angles
 // include/spins.h:25-32 or include/spins_sparse.h:34-41).
 // With a shared pair schedule (gkey != 0) the two pair uniforms come from
@@ -128,14 +140,7 @@ __device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t gkey, uint6
         x.y = z.y;
     }
     StepPre p;
-    double u1 = u32_to_unit(x.x), u2 = u32_to_unit(x.y);
-    int ri = (int)(u1 * N);
-    if (ri >= N) ri = N - 1;
-    int rj = (int)(u2 * (N - 1));
-    if (rj >= N - 1) rj = N - 2;
-    if (rj >= ri) rj++;
-    p.i0 = ri < rj ? ri : rj;
-    p.j0 = ri < rj ? rj : ri;
+    pick_pair(x.x, x.y, N, p.i0, p.j0);
     double ua = u32_to_unit(x.z), up1 = u32_to_unit(x.w), up2 = u32_to_unit(y.x), uacc = u32_to_unit(y.y);
     // angles 2 pi u with u in (0, 1): sincospi needs no large-argument reduction
     if (CUBE) {
@@ -149,6 +154,45 @@ __device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t gkey, uint6
     // u < exp(-beta dE)  <=>  dE < -log(u)/beta  (src/mc.cu:289-296)
     p.thr = (beta == 0.0) ? INFINITY : -log(uacc) / beta;
     return p;
+}
+
+// make_step split over five waves (each wave's lanes take the steps): part
+// 0 the pair from the replica stream (unless gkey != 0) and the amplitude
+// split, parts 1 and 2 the two phases, part 3 the threshold, part 4 the
+// pair from the group stream (gkey != 0).  Every field gets the value
+// make_step computes, by the same operations; a wave's latency is one
+// Philox block and one trigonometric function instead of all of them.
+template <bool CUBE>
+__device__ __noinline__ void make_step_part(StepPre& p, int part, uint64_t seed, uint64_t gkey, uint64_t sweep,
+                                               int step, int N, double beta) {
+    const U4 ctr0 = U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep0};
+    const U4 ctr1 = U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep1};
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    if (part == 4) {
+        if (gkey) {
+            const U4 z = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrPair},
+                                       (uint32_t)gkey, (uint32_t)(gkey >> 32));
+            pick_pair(z.x, z.y, N, p.i0, p.j0);
+        }
+    } else if (part <= 1) {
+        const U4 x = philox4x32_10(ctr0, k0, k1);
+        if (part == 0) {
+            if (!gkey) pick_pair(x.x, x.y, N, p.i0, p.j0);
+            const double ua = u32_to_unit(x.z);
+            if (CUBE) {
+                p.c[0] = sqrt(1.0 - ua);
+                p.c[1] = sqrt(ua);
+            } else {
+                sincospi(2.0 * ua, &p.c[1], &p.c[0]);
+            }
+        } else {
+            sincospi(2.0 * u32_to_unit(x.w), &p.c[3], &p.c[2]);
+        }
+    } else {
+        const U4 y = philox4x32_10(ctr1, k0, k1);
+        if (part == 2) sincospi(2.0 * u32_to_unit(y.x), &p.c[5], &p.c[4]);
+        else p.thr = (beta == 0.0) ? INFINITY : -log(u32_to_unit(y.y)) / beta;
+    }
 }
 
 // Local-field contribution of the thread for the current step, LISTS layout.
@@ -373,7 +417,7 @@ static_assert(offsetof(Prop, oi) % 16 == 0 && offsetof(Prop, fi) % 16 == 0 && of
 // Constraint-preserving proposal (include/spins.h:17-52,
 // include/spins_sparse.h:26-56), FP64, from the current amplitudes ai, aj.
 template <bool CUBE>
-__device__ __forceinline__ Prop propose(const StepPre& P, double2 ai, double2 aj) {
+__device__ __noinline__ Prop propose(const StepPre& P, double2 ai, double2 aj) {
     const double r1 = ai.x * ai.x + ai.y * ai.y, r2 = aj.x * aj.x + aj.y * aj.y;
     double2 ni, nj;
     if (CUBE) {
@@ -457,7 +501,6 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     Prop(*prop)[2] = ss.prop;
     float(*part)[16] = ss.part;
     __shared__ int last_wg;
-    __shared__ unsigned int gen_seen;
     __shared__ int goff[kMaxN + 1];  // GROUPS: chunk offsets per site
     extern __shared__ float g2s[];  // [N][N] two-body couplings when g2_lds
 
@@ -481,7 +524,11 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     const uint64_t gkey = a.pair_groups > 0 ? pair_group_key(a.pair_seed, rep % a.pair_groups) : 0;
     double E = 0.0;
     long long nacc = 0, nprop = 0;
-    if (tid == 0) E = a.E[rep];
+    unsigned int gen0 = 0;  // grid-barrier generation before this launch's first bump
+    if (tid == 0) {
+        E = a.E[rep];
+        if (a.exchange && a.rounds > 1) gen0 = dev_load(a.done + 1);
+    }
     if (a.h2_active && a.g2_lds)
         for (int i = tid; i < N * N; i += bdim) g2s[i] = a.g2[i];
     if (GROUPS)
@@ -504,7 +551,8 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     for (int rd = 0; rd < a.rounds; rd++) {
     for (int sw = 0; sw < a.n_sweeps; sw++) {
         const uint64_t sweep = sweep0 + (uint64_t)rd * a.n_sweeps + (uint64_t)sw;
-        for (int k = tid; k < nsteps; k += bdim) pre[k] = make_step<CUBE>(seed, gkey, sweep, k, N, beta);
+        for (int part = w; part < 5; part += nw)  // five waves draw the steps together
+            for (int k = lane; k < nsteps; k += 64) make_step_part<CUBE>(pre[k], part, seed, gkey, sweep, k, N, beta);
         __syncthreads();
         if (PAIRS && nb > 0)
             pair_issue<PB>(a, bA, desc_at(desc, 0), __builtin_amdgcn_readfirstlane(pre[0].i0),
@@ -665,18 +713,19 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         // rounds of a cooperative launch the others wait for it (a grid barrier:
         // arrival count, then a generation word bumped after the exchange).
         // Ordering across workgroups follows the HIP memory model at agent
-        // scope: each arrival is a release RMW (so its energy store, and its
-        // read of the generation, come before it), the last arriver acquires
+        // scope: each arrival is a release RMW (so its energy store comes
+        // before it), the last arriver acquires
         // before staging the energies; the generation bump is a release RMW
         // after the exchange's stores (made ordered before it by the
         // workgroup barrier), and a waiter acquires once it sees the bump,
         // before reading its new beta.  The data itself moves by agent-scope
         // atomics (dev_load, dev_store), so no access is served from a stale
-        // cache line.
+        // cache line.  The generation is bumped once per round, so round rd
+        // waits for it to leave gen0 + rd (gen0 read at launch start, when no
+        // bump is pending): no read of it on the arrival path.
         const bool more = rd + 1 < a.rounds;
         if (tid == 0) {
             dev_store(a.E + rep, E);
-            gen_seen = dev_load(a.done + 1);
             last_wg = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
             if (last_wg) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
@@ -697,7 +746,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             if (tid == 0) {
                 // Bounded wait: every workgroup is resident (cooperative launch), so
                 // the release comes; past the bound the kernel flags it and goes on.
-                for (unsigned n = 0; dev_load(a.done + 1) == gen_seen; n++) {
+                for (unsigned n = 0; dev_load(a.done + 1) == gen0 + (unsigned)rd; n++) {
                     if (n == kBarrierSpinLimit) {
                         __hip_atomic_fetch_or(a.done + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
