@@ -45,7 +45,7 @@ __device__ __forceinline__ float2 f2(double2 v) { return make_float2((float)v.x,
 // The draws of one step of one chain (k_sweep's make_step with the cube
 // proposal's amplitude-independent factors; src/mc_sparse.cu:118-126,
 // include/spins_sparse.h:34-41).
-__device__ __forceinline__ BatchDraw batch_draw(uint64_t seed, uint64_t sweep, int step, int N) {
+__device__ __noinline__ BatchDraw batch_draw(uint64_t seed, uint64_t sweep, int step, int N) {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const U4 x = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep0}, k0, k1);
     const U4 y = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep1}, k0, k1);
