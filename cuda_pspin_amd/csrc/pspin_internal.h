@@ -268,7 +268,7 @@ struct pspin_state {
     double* d_series = nullptr;
     int series_cap = 0;
     long long* d_series_n = nullptr;
-    unsigned int* d_done = nullptr;  // [3]: workgroups arrived, barrier generation, barrier wait expired
+    unsigned int* d_done = nullptr;  // [3]: workgroups arrived, unused, barrier wait expired
 
     // step log
     ps::StepRec* d_log = nullptr;

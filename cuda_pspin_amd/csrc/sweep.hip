@@ -52,6 +52,9 @@ namespace ps {
 
 // Grid-barrier wait bound (s_sleep 2 iterations, about a second).
 constexpr unsigned kBarrierSpinLimit = 1u << 23;
+// What a workgroup leaves in its beta slot before arriving at the grid
+// barrier: a NaN bit pattern, never a beta (betas are finite and >= 0).
+constexpr unsigned long long kBetaMarker = 0x7FF8BE7A0DDC0DE5ull;
 
 struct SweepArgs {
     int N, nrep, h2_active, n_sweeps;
@@ -79,7 +82,7 @@ struct SweepArgs {
     int npad;
     // replica exchange by the last workgroup to finish (exchange != 0)
     int exchange;
-    unsigned int* done;     // [0] workgroups arrived, [1] barrier generation, [2] barrier wait expired
+    unsigned int* done;     // [0] workgroups arrived, [1] unused, [2] barrier wait expired
     ExArgs ex;
     const int4* tasks;        // row batches {slot | first << 1 | last << 2 | rows << 8, first pair, first row, tile offset}
     const int* wave_tasks;    // [nwaves+1] batch ranges per wave (at most 64 each)
@@ -162,9 +165,11 @@ __device__ __forceinline__ StepPre make_step(uint64_t seed, uint64_t gkey, uint6
 // pair from the group stream (gkey != 0).  Every field gets the value
 // make_step computes, by the same operations; a wave's latency is one
 // Philox block and one trigonometric function instead of all of them.
+// With defer, part 3 stores -log(u) and the caller divides by beta later
+// (thr_fixup), the same two operations.
 template <bool CUBE>
 __device__ __noinline__ void make_step_part(StepPre& p, int part, uint64_t seed, uint64_t gkey, uint64_t sweep,
-                                               int step, int N, double beta) {
+                                               int step, int N, double beta, bool defer) {
     const U4 ctr0 = U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep0};
     const U4 ctr1 = U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), (uint32_t)step, kCtrStep1};
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
@@ -191,8 +196,14 @@ __device__ __noinline__ void make_step_part(StepPre& p, int part, uint64_t seed,
     } else {
         const U4 y = philox4x32_10(ctr1, k0, k1);
         if (part == 2) sincospi(2.0 * u32_to_unit(y.x), &p.c[5], &p.c[4]);
+        else if (defer) p.thr = -log(u32_to_unit(y.y));
         else p.thr = (beta == 0.0) ? INFINITY : -log(u32_to_unit(y.y)) / beta;
     }
+}
+
+// The threshold of a step drawn with defer, once beta is known.
+__device__ __forceinline__ void thr_fixup(StepPre& p, double beta) {
+    p.thr = (beta == 0.0) ? INFINITY : p.thr / beta;
 }
 
 // Local-field contribution of the thread for the current step, LISTS layout.
@@ -501,6 +512,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     Prop(*prop)[2] = ss.prop;
     float(*part)[16] = ss.part;
     __shared__ int last_wg;
+    __shared__ double beta_new;  // the beta after an exchange, read by thread 0
     __shared__ int goff[kMaxN + 1];  // GROUPS: chunk offsets per site
     extern __shared__ float g2s[];  // [N][N] two-body couplings when g2_lds
 
@@ -524,11 +536,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     const uint64_t gkey = a.pair_groups > 0 ? pair_group_key(a.pair_seed, rep % a.pair_groups) : 0;
     double E = 0.0;
     long long nacc = 0, nprop = 0;
-    unsigned int gen0 = 0;  // grid-barrier generation before this launch's first bump
-    if (tid == 0) {
-        E = a.E[rep];
-        if (a.exchange && a.rounds > 1) gen0 = dev_load(a.done + 1);
-    }
+    if (tid == 0) E = a.E[rep];
     if (a.h2_active && a.g2_lds)
         for (int i = tid; i < N * N; i += bdim) g2s[i] = a.g2[i];
     if (GROUPS)
@@ -548,11 +556,21 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         if (lane < nb) desc = a.tasks[t0 + lane];
     }
 
+    // A round's first sweep is drawn during the previous round's exchange
+    // (waves 1.. while wave 0 runs it or waits for it), with the thresholds
+    // left undivided until the new beta is known.
+    bool drawn = false;
     for (int rd = 0; rd < a.rounds; rd++) {
     for (int sw = 0; sw < a.n_sweeps; sw++) {
         const uint64_t sweep = sweep0 + (uint64_t)rd * a.n_sweeps + (uint64_t)sw;
-        for (int part = w; part < 5; part += nw)  // five waves draw the steps together
-            for (int k = lane; k < nsteps; k += 64) make_step_part<CUBE>(pre[k], part, seed, gkey, sweep, k, N, beta);
+        if (drawn) {  // the steps were drawn during the exchange; only the thresholds need beta
+            for (int k = tid; k < nsteps; k += bdim) thr_fixup(pre[k], beta);
+        } else {
+            for (int part = w; part < 5; part += nw)  // five waves draw the steps together
+                for (int k = lane; k < nsteps; k += 64)
+                    make_step_part<CUBE>(pre[k], part, seed, gkey, sweep, k, N, beta, false);
+        }
+        drawn = false;
         __syncthreads();
         if (PAIRS && nb > 0)
             pair_issue<PB>(a, bA, desc_at(desc, 0), __builtin_amdgcn_readfirstlane(pre[0].i0),
@@ -710,22 +728,31 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     }
     if (a.exchange) {
         // Last workgroup out runs the replica exchange on every energy.  Between
-        // rounds of a cooperative launch the others wait for it (a grid barrier:
-        // arrival count, then a generation word bumped after the exchange).
+        // rounds of a cooperative launch the others wait for their new beta
+        // (a grid barrier: an arrival count, and each replica's beta slot).
         // Ordering across workgroups follows the HIP memory model at agent
-        // scope: each arrival is a release RMW (so its energy store comes
-        // before it), the last arriver acquires
-        // before staging the energies; the generation bump is a release RMW
-        // after the exchange's stores (made ordered before it by the
-        // workgroup barrier), and a waiter acquires once it sees the bump,
-        // before reading its new beta.  The data itself moves by agent-scope
-        // atomics (dev_load, dev_store), so no access is served from a stale
-        // cache line.  The generation is bumped once per round, so round rd
-        // waits for it to leave gen0 + rd (gen0 read at launch start, when no
-        // bump is pending): no read of it on the arrival path.
+        // scope: each arrival is a release RMW, so the workgroup's energy
+        // store and the marker it leaves in its beta slot come before it; the
+        // last arriver acquires before staging the energies, so its stores of
+        // every replica's new beta come after every marker.  A waiter polls
+        // its own slot until the marker is gone and takes the value it reads
+        // as its beta: one read, and no other data from the exchange is used
+        // by it (the next exchanging workgroup reads the permutation after its
+        // own acquire, which the previous exchanger's next arrival releases).
+        // The data moves by agent-scope atomics (dev_load, dev_store), so no
+        // access is served from a stale cache line.
         const bool more = rd + 1 < a.rounds;
+        // the next round's first sweep, drawn by waves 1.. (nothing when the
+        // workgroup is one wave)
+        auto draw_next = [&]() {
+            const uint64_t sweep_n = sweep0 + (uint64_t)(rd + 1) * a.n_sweeps;
+            for (int part = w - 1; part < 5; part += nw - 1)
+                for (int k = lane; k < nsteps; k += 64)
+                    make_step_part<CUBE>(pre[k], part, seed, gkey, sweep_n, k, N, beta, true);
+        };
         if (tid == 0) {
             dev_store(a.E + rep, E);
+            if (more) dev_store((unsigned long long*)(a.betas + rep), kBetaMarker);
             last_wg = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
             if (last_wg) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
@@ -735,9 +762,10 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             exchange_stage(a.ex, &view[0][0], tid, bdim);
             __syncthreads();
             if (w == 0) exchange_wave(a.ex, &view[0][0], lane);
+            else if (more) draw_next();
             if (more) {
-                __syncthreads();  // wave 0's exchange stores happen before the release below
-                if (tid == 0) __hip_atomic_fetch_add(a.done + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __syncthreads();  // wave 0's beta stores are done: read back this replica's below
+                if (tid == 0) beta_new = dev_load(a.betas + rep);
                 // the exchange staged its slots over the amplitude copies
                 for (int i = lane; i < ((N + 3) & ~3); i += 64)
                     s32[i] = i < N ? make_float2((float)s64[i].x, (float)s64[i].y) : make_float2(0.f, 0.f);
@@ -745,20 +773,30 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         } else if (more) {
             if (tid == 0) {
                 // Bounded wait: every workgroup is resident (cooperative launch), so
-                // the release comes; past the bound the kernel flags it and goes on.
-                for (unsigned n = 0; dev_load(a.done + 1) == gen0 + (unsigned)rd; n++) {
+                // the new beta comes; past the bound the kernel flags it and keeps
+                // the old beta.
+                double b = beta;
+                for (unsigned n = 0;; n++) {
+                    const unsigned long long v = dev_load((const unsigned long long*)(a.betas + rep));
+                    if (v != kBetaMarker) {
+                        b = __longlong_as_double((long long)v);
+                        break;
+                    }
                     if (n == kBarrierSpinLimit) {
                         __hip_atomic_fetch_or(a.done + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(2);
                 }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                beta_new = b;
+            } else if (w > 0) {
+                draw_next();
             }
         }
         if (more) {
             __syncthreads();
-            beta = dev_load(a.betas + rep);  // this replica's temperature after the exchange
+            beta = beta_new;  // this replica's temperature after the exchange
+            drawn = nw > 1;
         }
     }
     }
