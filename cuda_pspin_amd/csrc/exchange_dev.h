@@ -173,6 +173,9 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
         pn = L.sP[t2 * nr + lr];
         en = L.sE[t2 * nr + lr];
         bn = L.sB[t2];
+        // keep these reads issued here: the end of the level copies them into
+        // the next level's registers, and waits for them there
+        __builtin_amdgcn_sched_barrier(0);
         const double delta = (bt - b1) * (ec - eb);
         const bool need = valid && !(delta >= 0.0);
         const unsigned long long m = __ballot(need);
@@ -187,8 +190,10 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
         const double dd = delta - (double)L.sLU[k];
         // (bitwise: no short-circuit branches)
         bool accept = valid & (!need | (dd > 3e-5));
-        const bool band = need & ((delta < -700.0) | !((dd > 3e-5) | (dd < -3e-5)));
-        if (__ballot(band)) {  // about 6e-5 of draws: the decision in FP64 on the draw itself
+        const bool band = need & ((delta < -700.0) | (fabs(dd) <= 3e-5));
+        // about 6e-5 of draws: the decision in FP64 on the draw itself (kept
+        // off the fall-through path, which every level takes)
+        if (__builtin_expect(__ballot(band) != 0, 0)) {
             if (band) {
                 uint64_t A, C;
                 lcg48_coeffs((uint64_t)k + 1, &A, &C);
@@ -205,7 +210,7 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
         ec = accept ? ec : eb;
         bt = b1;
         abits |= (unsigned long long)accept << (t & 63);
-        if ((t & 63) == 63 || t == NT - 2) {  // uniform: every 64 levels and at the end
+        if (__builtin_expect((t & 63) == 63 || t == NT - 2, 0)) {  // uniform: every 64 levels and at the end
             L.sAB[(t >> 6) * 64 + lane] = abits;
             abits = 0;
         }
@@ -214,6 +219,10 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // the betas first: in a multi-round launch every other workgroup is
+    // waiting for its own
+    for (int i = lane; i < n; i += 64) dev_store(a.betas + L.sP[i], L.sB[i / nr]);
+    for (int i = lane; i < n; i += 64) dev_store(a.perm + i, L.sP[i]);
     for (int t = lane; t < NT - 1; t += 64) {  // accepted swaps per level
         const unsigned long long* w = L.sAB + (t >> 6) * 64;
         long long c = 0;
@@ -223,11 +232,6 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int i = lane; i < n; i += 64) {
-        const int p = L.sP[i];
-        dev_store(a.perm + i, p);
-        dev_store(a.betas + p, L.sB[i / nr]);
-    }
     for (int t = lane; t < NT - 1; t += 64) {  // the counters as staged, plus this round
         long long* ex = a.ex;
         const long long k = L.sacc[t];
