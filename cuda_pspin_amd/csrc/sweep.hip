@@ -310,12 +310,12 @@ __device__ __forceinline__ float pair_step(const SweepArgs& a, PairBuf<PB>& B, i
                                            int lane) {
     const int nr = (bd.x >> 8) & 15, nn = (bn.x >> 8) & 15;
     const int nsite = (bn.x & 1) ? nj : ni;
-    if (bd.x & 2) {
+    if (__builtin_expect(bd.x & 2, 0)) {  // a segment's first batch (about a third of them)
         inf = B.inf;
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[j] = make_float2(0.f, 0.f);
     }
-    if (bn.x & 2) B.inf = *(const uint4*)(a.pinfo + (size_t)nsite * a.npad + bn.y + 4 * lane);
+    if (__builtin_expect(bn.x & 2, 0)) B.inf = *(const uint4*)(a.pinfo + (size_t)nsite * a.npad + bn.y + 4 * lane);
     const float4* tp = (const float4*)(s32 + bd.z);
 #pragma unroll
     for (int u = 0; u < PB; u += 4) {
@@ -334,7 +334,7 @@ __device__ __forceinline__ float pair_step(const SweepArgs& a, PairBuf<PB>& B, i
         }
         if (u < nn) pair_load_quad<PB>(a, B, bn, nsite, lane, u);
     }
-    if (!(bd.x & 4)) return 0.f;
+    if (__builtin_expect(!(bd.x & 4), 1)) return 0.f;  // not a segment's last batch
     const uint32_t e[4] = {inf.x, inf.y, inf.z, inf.w};
     float2 Hc = make_float2(0.f, 0.f);
 #pragma unroll
@@ -619,7 +619,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 const bool accept = (beta == 0.0) | ((double)dE < thr);
                 s32[ri] = accept ? make_float2(f01.x, f01.y) : make_float2(o01.x, o01.y);
                 s32[rj] = accept ? make_float2(f01.z, f01.w) : make_float2(o01.z, o01.w);
-                if (w == 0 && accept) {
+                if (__builtin_expect(w == 0 && accept, 0)) {
                     const double2 ni = Q.ni, nj = Q.nj;
                     s64[ri] = ni;
                     s64[rj] = nj;
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                         nacc++;
                     }
                     nprop++;
-                    if (logp) {
+                    if (__builtin_expect(logp != nullptr, 0)) {
                         StepRec r;
                         r.i0 = ri; r.j0 = rj; r.acc = accept; r.dE = dE;
                         r.ai[0] = Q.ni.x; r.ai[1] = Q.ni.y;
@@ -651,7 +651,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             s32[j0] = make_float2(m01.z, m01.w);
             const Prop& Qs = prop[r_st][hyp];
             float local = 0.f;
-            if (a.h2_active && w == nw - 1) {
+            if (__builtin_expect(a.h2_active && w == nw - 1, 0)) {
                 float2 Fi = make_float2(0.f, 0.f), Fj = make_float2(0.f, 0.f);
                 if (a.g2_lds) {  // no global load here: it would wait behind the prefetched rows
                     for (int k = lane; k < N; k += 64) {
@@ -709,7 +709,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 local += field_lists<BF16, PB>(a, s32, i0, j0, di, dj, tid, bdim);
             local = wave_sum(local);
             if (lane == 0) part[step & 1][w] = local;
-            if (w == 0 && lane < 2 && step + 1 < nsteps) {
+            if (__builtin_expect(w == 0 && lane < 2 && step + 1 < nsteps, 0)) {
                 // Next step's proposal if this step is rejected (lane 0) or accepted (lane 1).
                 const StepPre& Pn = pre[step + 1];
                 double2 ai = s64[Pn.i0], aj = s64[Pn.j0];
