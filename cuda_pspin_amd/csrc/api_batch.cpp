@@ -44,8 +44,6 @@ struct pspin_batch {
     long long *d_acc = nullptr, *d_prop = nullptr, *d_ex = nullptr, *d_done = nullptr;
     uint64_t *d_seeds = nullptr, *d_ctr = nullptr, *d_rng48 = nullptr;
     int *d_since = nullptr, *d_perm = nullptr;
-    ps::BatchDraw* d_draws = nullptr;  // per-sweep draw scratch
-    int max_steps = 1;
     int resync_every = 1024;
     int chains_per_sample = 1;  // NT * nrep_per_temp (picks the lanes per chain)
     size_t lds = 0;
@@ -78,7 +76,7 @@ void batch_free(pspin_batch* b) {
     dfree(b->d_samples); dfree(b->d_e4); dfree(b->d_e2); dfree(b->d_o4); dfree(b->d_o2); dfree(b->d_q);
     dfree(b->d_qg); dfree(b->d_h2g); dfree(b->d_h2ij); dfree(b->d_spins); dfree(b->d_E); dfree(b->d_betas);
     dfree(b->d_sched); dfree(b->d_acc); dfree(b->d_prop); dfree(b->d_ex); dfree(b->d_done); dfree(b->d_seeds);
-    dfree(b->d_ctr); dfree(b->d_rng48); dfree(b->d_since); dfree(b->d_perm); dfree(b->d_draws);
+    dfree(b->d_ctr); dfree(b->d_rng48); dfree(b->d_since); dfree(b->d_perm);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -105,7 +103,6 @@ int batch_args(pspin_batch* b, ps::BatchArgs& a, int n_sweeps, int pt_freq) {
     a.spins = b->d_spins; a.E = b->d_E; a.acc = b->d_acc; a.prop = b->d_prop; a.seeds = b->d_seeds;
     a.ctr = b->d_ctr; a.since = b->d_since; a.betas = b->d_betas; a.perm = b->d_perm;
     a.beta_sched = b->d_sched; a.ex = b->d_ex; a.rng48 = b->d_rng48; a.done = b->d_done;
-    a.draws = b->d_draws; a.max_steps = b->max_steps;
     return PSPIN_OK;
 }
 
@@ -235,9 +232,8 @@ int pspin_batch_create(const pspin_sim_config* base, int n_samples, const int* N
     }
     PB_REQUIRE(lds_max <= 160 * 1024, PSPIN_ERR_ARG, "a sample needs more than 160 KB of LDS (N * chains too large)");
     b->lds = lds_max;
-    // the draws of a sweep are made by every thread; each chain runs on 4 or 8 lanes
+    // each chain runs on 4 or 8 lanes
     b->chains_per_sample = C;
-    for (const ps::BatchSample& S : b->h_samples) b->max_steps = std::max(b->max_steps, S.N / 2);
     hipStream_t s = b->stream;
     const size_t nch = (size_t)b->chains;
     std::vector<uint64_t> zero64(nch, 0);
@@ -267,7 +263,6 @@ int pspin_batch_create(const pspin_sim_config* base, int n_samples, const int* N
     PB_ALLOC(b->d_rng48, rng48.size(), rng48.data());
     PB_ALLOC(b->d_since, nch, zero32.data());
     PB_ALLOC(b->d_perm, perm.size(), perm.data());
-    PB_ALLOC(b->d_draws, nch * (size_t)b->max_steps, nullptr);
 #undef PB_ALLOC
     b->n_spin = spins.size();
     b->n_perm = perm.size();

@@ -16,17 +16,23 @@
 // order and drand48 stream for that sample, and every chain picks up its new
 // beta.
 //
-// A sweep runs in two phases.  First the workgroup makes the state-independent
-// draws of all (step, chain) pairs of the sample (Philox blocks, pair, phases,
-// intensity split, -log u of the threshold) into a global scratch array,
-// [step][chain]; these FP64 transcendentals are most of a step's
-// instructions, and this way they run with the whole workgroup's waves hiding
-// their latency.  Then each chain's lanes walk its steps, prefetching the next
-// step's draws, with only the amplitude-dependent part left: three square
-// roots of the cube proposal, the local fields and the decision.  The draws
-// do not depend on beta, so after a sweep that ends in an exchange, wave 0
-// runs the (serial) exchange while the other waves make the next sweep's
-// draws.
+// Draws.  The state-independent part of a step (Philox blocks, pair,
+// phases, intensity split, -log u of the threshold) is mostly FP64
+// transcendentals.  Each lane of a chain makes one step's draws in L: lane
+// `sub` holds, in registers, the draws of step b L + sub of the current
+// block b, and the chain's lanes take step t's draws from lane t mod L of
+// their group by ds_bpermute.  When a block's last step starts, every lane
+// makes its step of the next block (or of the next sweep's first block: the
+// draws do not depend on beta), so the draw work is spread over the L lanes
+// and interleaved with the chains' LDS-latency-bound term loops, and no draw
+// leaves the registers.
+//
+// Exchange.  After a sweep that ends in an exchange, wave 0 runs it with the
+// reference's decision order and drand48 stream for that sample
+// (src/parallel_tempering.cu:316-352): the logs of every draw the round can
+// take are made in parallel first (the stream's jump-ahead), so the serial
+// level chain holds no exponential and no 48-bit product; a decision within
+// a narrow band of its draw's log is settled in FP64 on the draw itself.
 //
 // Per chain the arithmetic is the dense kernel's: FP64 cube proposal
 // (include/spins_sparse.h:26-56), FP32 midpoint local-field Delta E
@@ -79,6 +85,188 @@ __device__ __forceinline__ float chain_sum(float v) {
     return v;
 }
 
+// The kernel's LDS for one sample (offsets from batch_lds_layout).
+struct BatchLds {
+    double2* amp;  // [N][C | 1]
+    uint2* e4;
+    uint2* e2;
+    int* o4;
+    int* o2;
+    double* sE;
+    double* sES;
+    double* sB;
+    int* sperm;
+    uint64_t* jA;
+    uint64_t* jC;
+    long long* sacc;
+    double* sBs;
+    double* sDb;
+    unsigned long long* sAB;
+    float* sLU;
+    __device__ BatchLds(void* lds, const BatchSample& s, int NT) {
+        const BatchLdsLayout L = batch_lds_layout(s, NT);
+        char* b = (char*)lds;
+        amp = (double2*)(b + L.amp);
+        e4 = (uint2*)(b + L.e4);
+        e2 = (uint2*)(b + L.e2);
+        o4 = (int*)(b + L.o4);
+        o2 = (int*)(b + L.o2);
+        sE = (double*)(b + L.sE);
+        sES = (double*)(b + L.sES);
+        sB = (double*)(b + L.sB);
+        sperm = (int*)(b + L.sperm);
+        jA = (uint64_t*)(b + L.jA);
+        jC = (uint64_t*)(b + L.jC);
+        sacc = (long long*)(b + L.sacc);
+        sBs = (double*)(b + L.sBs);
+        sDb = (double*)(b + L.sDb);
+        sAB = (unsigned long long*)(b + L.sAB);
+        sLU = (float*)(b + L.sLU);
+    }
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One exchange round of a sample with nr <= 64 replicas per level, on one
+// wave (src/parallel_tempering.cu:316-352: levels from the hottest, columns
+// in order, a draw only for delta < 0).  Lane r carries the chain that climbs
+// column r: at level t it holds the chain that levels < t left at (t, r).
+// Slot t+2's chain and energy are read ahead, off the serial path; a level's
+// draw is found by its rank among the round's draws.  Returns the drand48
+// state after the round.
+__device__ __forceinline__ uint64_t batch_exchange_chain(const BatchLds& B, uint64_t X0, int lane, int NT, int nr) {
+    const int n = NT * nr, ndraw = nr * (NT - 1);
+    const bool valid = lane < nr;
+    // energies in slot order, and logf of every draw the round can take
+    // (draw k is X_{k+1}: lane l jumps l + 1 steps, then 64 at a time)
+    for (int i = lane; i < n; i += 64) B.sES[i] = B.sE[B.sperm[i]];
+    {
+        uint64_t x = (B.jA[lane + 1] * X0 + B.jC[lane + 1]) & kMask48;
+        const uint64_t A64 = B.jA[64], C64 = B.jC[64];
+        for (int k = lane; k < ndraw; k += 64) {
+            B.sLU[k] = logf((float)lcg48_unit(x));
+            x = (A64 * x + C64) & kMask48;
+        }
+    }
+    wave_sync_lds();
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int lr = valid ? lane : nr - 1;  // lanes past the ladders follow the last column, unused
+    int pc = B.sperm[lr];                  // chain carried down the column
+    double ec = B.sES[lr];                 // its energy
+    int pn = B.sperm[nr + lr];             // slot t+1 before level t
+    double en = B.sES[nr + lr];
+    double dn = B.sDb[0];
+    int base = 0;                  // draws taken so far
+    unsigned long long abits = 0;  // this lane's accepts, bit t % 64
+    for (int t = 0; t < NT - 1; t++) {
+        const int pb = pn;
+        const double eb = en, db = dn;
+        const int t2 = t + 2 < NT ? t + 2 : NT - 1;
+        pn = B.sperm[t2 * nr + lr];
+        en = B.sES[t2 * nr + lr];
+        dn = B.sDb[t + 1 < NT - 1 ? t + 1 : t];
+        __builtin_amdgcn_sched_barrier(0);  // the next level's reads go out here
+        const double delta = db * (ec - eb);
+        const bool need = valid && !(delta >= 0.0);
+        const unsigned long long m = __ballot(need);
+        // rank of this lane's draw (below ndraw for every lane that needs one;
+        // at most ndraw, an entry of the array, for the others)
+        const int k = base + __popcll(m & lt);
+        // u < exp(delta) from lu = logf((float)u), |lu - log u| < 1e-5: outside a
+        // 3e-5 band around delta the order of lu and delta is that of u and
+        // exp(delta).  Below delta = -700 (and for u = 0) the FP64 test decides.
+        const double dd = delta - (double)B.sLU[k];
+        bool accept = valid & (!need | (dd > 3e-5));
+        const bool band = need & ((delta < -700.0) | (fabs(dd) <= 3e-5));
+        if (__builtin_expect(__ballot(band) != 0, 0)) {
+            if (band) {
+                uint64_t A, Cc;
+                lcg48_coeffs((uint64_t)k + 1, &A, &Cc);
+                accept = lcg48_unit((A * X0 + Cc) & kMask48) < exp(delta);
+            }
+        }
+        base += __popcll(m);
+        // slot t keeps the carried chain unless the pair swaps; a swap carries it on
+        B.sperm[valid ? t * nr + lane : n] = accept ? pb : pc;
+        pc = accept ? pc : pb;
+        ec = accept ? ec : eb;
+        abits |= (unsigned long long)accept << (t & 63);
+        if (__builtin_expect((t & 63) == 63 || t == NT - 2, 0)) {
+            B.sAB[(t >> 6) * 64 + lane] = abits;
+            abits = 0;
+        }
+    }
+    if (valid) B.sperm[(NT - 1) * nr + lane] = pc;
+    wave_sync_lds();
+    for (int t = lane; t < NT - 1; t += 64) {  // accepted swaps per level
+        const unsigned long long* w = B.sAB + (t >> 6) * 64;
+        long long cnt = 0;
+        for (int r = 0; r < nr; r++) cnt += (long long)((w[r] >> (t & 63)) & 1ull);
+        B.sacc[t] += cnt;
+    }
+    uint64_t A, Cc;
+    lcg48_coeffs((uint64_t)base, &A, &Cc);
+    return (A * X0 + Cc) & kMask48;
+}
+
+// The same round for more than 64 replicas per level: columns in blocks of
+// 64 lanes, permutation entries swapped in LDS.
+__device__ __forceinline__ uint64_t batch_exchange_wide(const BatchLds& B, uint64_t X, int lane, int NT, int nr) {
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int t = 0; t < NT - 1; t++) {
+        long long na = 0;
+        for (int c0 = 0; c0 < nr; c0 += 64) {
+            const int r = c0 + lane;
+            const bool valid = r < nr;
+            double delta = 0.0;
+            if (valid) delta = B.sDb[t] * (B.sE[B.sperm[t * nr + r]] - B.sE[B.sperm[(t + 1) * nr + r]]);
+            const bool need = valid && !(delta >= 0.0);
+            const unsigned long long m = __ballot(need);
+            bool accept = valid;
+            if (need) {
+                const int k = __popcll(m & lt) + 1;
+                accept = lcg48_unit((B.jA[k] * X + B.jC[k]) & kMask48) < exp(delta);
+            }
+            const int tot = __popcll(m);
+            X = (B.jA[tot] * X + B.jC[tot]) & kMask48;
+            if (valid && accept) {
+                const int tmp = B.sperm[t * nr + r];
+                B.sperm[t * nr + r] = B.sperm[(t + 1) * nr + r];
+                B.sperm[(t + 1) * nr + r] = tmp;
+            }
+            na += __popcll(__ballot(valid && accept));
+        }
+        if (lane == 0) B.sacc[t] += na;
+    }
+    wave_sync_lds();
+    return X;
+}
+
+// Lane addr / 4's value of v (ds_bpermute; the source lane is active).
+__device__ __forceinline__ double bperm_f64(int addr, double v) {
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ BatchDraw bperm_draw(int addr, const BatchDraw& o) {
+    BatchDraw d;
+    d.ij = __builtin_amdgcn_ds_bpermute(addr, o.ij);
+    d.pad = 0;
+    d.c1 = bperm_f64(addr, o.c1);
+    d.s1 = bperm_f64(addr, o.s1);
+    d.c2 = bperm_f64(addr, o.c2);
+    d.s2 = bperm_f64(addr, o.s2);
+    d.q1 = bperm_f64(addr, o.q1);
+    d.q2 = bperm_f64(addr, o.q2);
+    d.lu = bperm_f64(addr, o.lu);
+    return d;
+}
+
 // LDS layout (dynamic): FP64 amplitudes [N][P] with P = C | 1 (site-major so
 // the chains of one site are adjacent; the odd pitch puts the different sites
 // that the lanes of one chain read in different banks), H4 entries, H2
@@ -90,7 +278,7 @@ __device__ __forceinline__ float chain_sum(float v) {
 // those of site j, and chain_sum adds the L partial Delta E.  Lane sub = 0
 // writes the accepted amplitudes and owns the chain's counters and energy.
 // The design point is one workgroup per CU (a sample at N = 64 with 96 chains
-// takes about 104 KB of LDS), so a SIMD holds TB / 256 of its waves and the
+// takes about 106 KB of LDS), so a SIMD holds TB / 256 of its waves and the
 // compiler may use the registers that leaves.
 template <int L, int TB>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256))) void k_batch(BatchArgs a) {
@@ -101,19 +289,20 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
     constexpr int H = L / 2;
     const int c = tid / L, sub = tid % L, half = sub / H, hs = sub % H;
     const bool live = c < C, lead = live && sub == 0;
-    double2* amp = (double2*)lds;                       // [N][P]
-    uint2* e4 = (uint2*)(amp + (size_t)N * P);          // H4 entries
-    uint2* e2 = e4 + S.n_e4;                            // H2 entries {k, g}
-    int* o4 = (int*)(e2 + S.n_e2);                      // [N+1]
-    int* o2 = o4 + N + 1;                               // [N+1]
-    double* sE = (double*)(((uintptr_t)(o2 + N + 1) + 7) & ~(uintptr_t)7);  // [C]
-    double* sB = sE + C;                                // [C] beta by chain
-    int* sperm = (int*)(sB + C);                        // [C]
-    uint64_t* jA = (uint64_t*)(((uintptr_t)(sperm + C) + 7) & ~(uintptr_t)7);  // [65]
-    uint64_t* jC = jA + 65;
-    long long* sacc = (long long*)(jC + 65);            // [NT] accepted swaps per level over the launch
-    double* sBs = (double*)(sacc + NT);                 // [NT] beta schedule
-    double* sDb = sBs + NT;                             // [NT] beta_t - beta_{t+1}
+    const BatchLds B(lds, S, NT);
+    double2* amp = B.amp;
+    uint2* e4 = B.e4;
+    uint2* e2 = B.e2;
+    int* o4 = B.o4;
+    int* o2 = B.o2;
+    double* sE = B.sE;
+    double* sB = B.sB;
+    int* sperm = B.sperm;
+    uint64_t* jA = B.jA;
+    uint64_t* jC = B.jC;
+    long long* sacc = B.sacc;
+    double* sBs = B.sBs;
+    double* sDb = B.sDb;
 
     // ---- stage the sample's terms and the chains' amplitudes
     for (int i = tid; i < S.n_e4; i += blockDim.x) e4[i] = a.e4[S.e4_off + i];
@@ -142,31 +331,26 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
     double beta = live ? a.betas[ch] : 0.0;
     double E = live ? a.E[ch] : 0.0;
     uint64_t sweep = live ? a.ctr[ch] : 0;
+    const uint64_t cseed = live ? a.seeds[ch] : 0;
     long long nacc = 0, nprop = 0, nex = 0;
     int since = live ? a.since[ch] : 0;
     uint64_t X = (tid < 64) ? a.rng48[blockIdx.x] : 0;  // wave 0 keeps the sample's drand48 state
     const long long done0 = a.done[blockIdx.x];
-    __syncthreads();
 
     const int nsteps = N / 2;
-    BatchDraw* dr = a.draws + (size_t)S.chain_off * a.max_steps;  // this sample's [step][chain] draws
-    // Phase 1 of sweep sw: every (step, chain) draw, by threads [first, blockDim.x).
-    auto make_draws = [&](int sw, int first) {
-        for (int x = tid - first; x < nsteps * C; x += (int)blockDim.x - first) {
-            const int st = x / C, cx = x - st * C;
-            const int gc = S.chain_off + cx;
-            dr[x] = batch_draw(a.seeds[gc], a.ctr[gc] + (uint64_t)sw, st, N);
-        }
-    };
-    if (a.n_sweeps > 0) make_draws(0, 0);
+    // This lane's draws: step blk * L + sub of the current block.  The first
+    // block of the launch's first sweep is made before the staging barrier.
+    BatchDraw own{};
+    if (live && a.n_sweeps > 0) own = batch_draw(cseed, sweep, sub, N);
+    const int gaddr = (lane & ~(L - 1)) * 4;  // ds_bpermute address of lane 0 of this chain's group
     __syncthreads();
+
     for (int sw = 0; sw < a.n_sweeps; sw++) {
-        // Phase 2: each chain's steps, the next step's draws loaded one ahead.
         if (live) {
-            BatchDraw nd = dr[c];
             for (int step = 0; step < nsteps; step++) {
-                const BatchDraw d = nd;
-                if (step + 1 < nsteps) nd = dr[(size_t)(step + 1) * C + c];
+                const int q = step & (L - 1);
+                const bool blk_end = q == L - 1 || step == nsteps - 1;  // uniform
+                const BatchDraw d = bperm_draw(gaddr + q * 4, own);
                 const int i0 = d.ij & 0xffff, j0 = d.ij >> 16;
                 // this lane's site (i for the first half of the chain's lanes, j for the second);
                 // its list bounds and first entries are read before the proposal's arithmetic
@@ -230,6 +414,15 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
                     E += (double)dE;
                     nacc++;
                 }
+                if (blk_end) {
+                    // every lane makes its step of the next block: the next L steps of
+                    // this sweep, or the first L of the next sweep (steps past the
+                    // sweep's end are made and never read; nothing that is live
+                    // across the call is left but the chain's state)
+                    const bool next_sw = step + 1 >= nsteps;
+                    if (!next_sw || sw + 1 < a.n_sweeps)
+                        own = batch_draw(cseed, next_sw ? sweep + 1 : sweep, (next_sw ? 0 : step + 1) + sub, N);
+                }
             }
             sweep++;
             since++;
@@ -260,85 +453,20 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, TB / 256)
             }
         }
         const bool exch = a.pt_freq > 0 && (done0 + sw + 1) % a.pt_freq == 0;
-        if (exch && lead) sE[c] = E;
-        __syncthreads();  // the chains are done with this sweep's draws and have published their energies
-        // Wave 0 runs the exchange while the other waves make the next sweep's draws.
-        if (sw + 1 < a.n_sweeps && (!exch || tid >= 64)) make_draws(sw + 1, exch ? 64 : 0);
-        if (exch) {
-            // Exchange (src/parallel_tempering.cu:316-352) for this sample.
-            if (tid < 64) {
-                const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-                if (nr <= 64) {
-                    // Lane r carries the chain that climbs column r: at level t it
-                    // holds the chain that levels < t left at (t, r).  The chain at
-                    // (t + 1, r) is not written before level t, so it and its
-                    // energy are read a level ahead, off the serial path.
-                    const bool valid = lane < nr;
-                    int cur = valid ? sperm[lane] : 0;
-                    double Ecur = valid ? sE[cur] : 0.0;
-                    int nx = valid ? sperm[nr + lane] : 0;
-                    double Enx = valid ? sE[nx] : 0.0;
-                    for (int t = 0; t < NT - 1; t++) {
-                        const int n1 = nx;
-                        const double E1 = Enx;
-                        if (valid && t + 2 < NT) {
-                            nx = sperm[(t + 2) * nr + lane];
-                            Enx = sE[nx];
-                        }
-                        const double delta = sDb[t] * (Ecur - E1);
-                        const bool need = valid && !(delta >= 0.0);
-                        const unsigned long long m = __ballot(need);
-                        bool accept = valid;
-                        if (need) {
-                            const int k = __popcll(m & lt) + 1;
-                            accept = lcg48_unit((jA[k] * X + jC[k]) & kMask48) < exp(delta);
-                        }
-                        const int tot = __popcll(m);
-                        X = (jA[tot] * X + jC[tot]) & kMask48;
-                        if (valid) {
-                            sperm[t * nr + lane] = accept ? n1 : cur;
-                            if (!accept) {
-                                cur = n1;
-                                Ecur = E1;
-                            }
-                        }
-                        const unsigned long long am = __ballot(valid && accept);
-                        if (lane == 0) sacc[t] += __popcll(am);
-                    }
-                    if (valid) sperm[(NT - 1) * nr + lane] = cur;
-                } else {
-                    for (int t = 0; t < NT - 1; t++) {
-                        long long na = 0;
-                        for (int c0 = 0; c0 < nr; c0 += 64) {
-                            const int r = c0 + lane;
-                            const bool valid = r < nr;
-                            double delta = 0.0;
-                            if (valid) delta = sDb[t] * (sE[sperm[t * nr + r]] - sE[sperm[(t + 1) * nr + r]]);
-                            const bool need = valid && !(delta >= 0.0);
-                            const unsigned long long m = __ballot(need);
-                            bool accept = valid;
-                            if (need) {
-                                const int k = __popcll(m & lt) + 1;
-                                accept = lcg48_unit((jA[k] * X + jC[k]) & kMask48) < exp(delta);
-                            }
-                            const int tot = __popcll(m);
-                            X = (jA[tot] * X + jC[tot]) & kMask48;
-                            if (valid && accept) {
-                                const int tmp = sperm[t * nr + r];
-                                sperm[t * nr + r] = sperm[(t + 1) * nr + r];
-                                sperm[(t + 1) * nr + r] = tmp;
-                            }
-                            na += __popcll(__ballot(valid && accept));
-                        }
-                        if (lane == 0) sacc[t] += na;
-                    }
-                }
-                for (int i = lane; i < C; i += 64) sB[sperm[i]] = sBs[i / nr];
-            }
-            nex++;
+        if (!exch) continue;
+        if (lead) sE[c] = E;
+        __syncthreads();  // every chain has published its energy
+        // Exchange (src/parallel_tempering.cu:316-352) for this sample, on wave 0.
+        if (tid < 64) {
+            if (nr <= 64)
+                X = batch_exchange_chain(B, X, lane, NT, nr);
+            else
+                X = batch_exchange_wide(B, X, lane, NT, nr);
+            for (int i = lane; i < C; i += 64) sB[sperm[i]] = sBs[i / nr];
         }
-        __syncthreads();  // the next sweep's draws are made and the new betas published
-        if (exch && live) beta = sB[c];
+        nex++;
+        __syncthreads();  // the new betas are published
+        if (live) beta = sB[c];
     }
 
     // ---- exchange counters of the launch, once
@@ -397,12 +525,7 @@ const void* batch_kernel(int chains) {
 }
 }  // namespace
 
-size_t batch_lds_bytes(const BatchSample& s, int NT) {
-    size_t b = (size_t)s.N * (s.chains | 1) * 16 + (size_t)(s.n_e4 + s.n_e2) * 8 + (size_t)2 * (s.N + 1) * 4;
-    b = (b + 7) / 8 * 8 + (size_t)s.chains * 16 + (size_t)s.chains * 4;
-    b = (b + 7) / 8 * 8 + 2 * 65 * 8 + (size_t)NT * 8 * 3;
-    return b;
-}
+size_t batch_lds_bytes(const BatchSample& s, int NT) { return batch_lds_layout(s, NT).end; }
 
 hipError_t batch_prepare(int chains, size_t lds) {
     // Dynamic LDS above 64 KB must be allowed explicitly for the kernel.
