@@ -8,7 +8,8 @@
   proposal densities, both energies, the MH log-ratio and the decision.
 * The stationary law: learned moves alone at beta = 0 give Dirichlet(1/2)
   intensities; the PT mixture with learned moves matches plain PT.
-* Training on PT snapshots lowers the loss and lengthens accepted moves.
+* Training on PT snapshots lowers the loss, lengthens accepted moves and
+  raises the energy ESS per move of chains of learned moves alone.
 """
 import math
 
@@ -261,5 +262,57 @@ def test_training_lowers_loss_and_lengthens_moves(pspin, oracle):
     after = esjd(40)
     print(f"  squared jump per move: initial network {before:.4f}, trained {after:.4f}")
     assert after > before
+    lp.close()
+    st.close()
+
+
+def test_training_speeds_energy_decorrelation(pspin, oracle):
+    """Energy ESS of chains of learned moves alone (no sweeps, no exchange),
+    the trained network against the initial one, on the same state, at the
+    benchmark's proposal scale (s = 0.3, mean scale s^2).
+
+    For a stationary energy series, 1 - rho_1 = E[(E' - E)^2] / (2 Var E), and
+    the ESS per sample (1 - rho_1) / (1 + rho_1) is close to (1 - rho_1) / 2
+    for rho_1 near 1.  Var E is the same equilibrium variance for both
+    networks (pooled over both runs), so the ratio of the mean squared energy
+    change per move is the ratio of the energy ESS per move.  Training on PT
+    snapshots must raise it at every temperature."""
+    NT, nr, N = 4, 16, 16
+    T, beta = oracle.temperatures(NT, 0.8, 2.5, 1)
+    st = _state(pspin, N=N, nrep=NT * nr, seed=13)
+    st.pt_init(NT, nr, beta, 9)
+    st.pt_run(300, 1)
+    lp = _lp(pspin, st, seed=5)
+    lp.set_scale(0.3)
+    lp.set_mean_scale(0.09)
+    lp.collect(64, 1, 1)
+    perm = st.pt_perm().reshape(NT, nr)  # moves alone do not exchange: slots stay fixed
+
+    def series(n_moves):
+        E = np.zeros((n_moves + 1, NT * nr))
+        E[0] = st.results()[0]
+        lp.reset_stats()
+        for k in range(n_moves):
+            lp.move()
+            E[k + 1] = st.results()[0]
+        a, p = lp.stats()
+        return E, a / np.maximum(p, 1)
+
+    E0, acc0 = series(600)
+    hist = lp.train(300, batch=512, lag=1, lr=1e-3)
+    assert hist[-10:].mean() < hist[:10].mean()
+    E1, acc1 = series(600)
+    ratio = np.zeros(NT)
+    for t in range(NT):
+        e0, e1 = E0[:, perm[t]], E1[:, perm[t]]
+        var = np.concatenate([e0 - e0.mean(axis=0), e1 - e1.mean(axis=0)]).var()
+        j0 = np.mean(np.diff(e0, axis=0) ** 2)
+        j1 = np.mean(np.diff(e1, axis=0) ** 2)
+        r0, r1 = 1 - j0 / (2 * var), 1 - j1 / (2 * var)
+        ratio[t] = ((1 - r1) / (1 + r1)) / ((1 - r0) / (1 + r0))
+        print(f"  T={T[t]:.3f}: rho1 initial {r0:.4f} (acc {acc0[t]:.3f}) trained {r1:.4f} "
+              f"(acc {acc1[t]:.3f}); energy ESS per move x{ratio[t]:.2f}")
+    assert np.all(ratio > 1.3), ratio
+    assert ratio.mean() > 1.5, ratio
     lp.close()
     st.close()
