@@ -80,7 +80,7 @@ void free_act(pspin_mlp_act& a) {
 void free_lp(pspin_learned* lp) {
     free_act(lp->tr); free_act(lp->fw); free_act(lp->rv);
     dfree(lp->d_p); dfree(lp->d_m); dfree(lp->d_v); dfree(lp->d_g); dfree(lp->d_pb);
-    dfree(lp->d_gpart); dfree(lp->d_bpart);
+    dfree(lp->d_gpart);
     dfree(lp->d_data); dfree(lp->d_dbeta);
     dfree(lp->d_u); dfree(lp->d_up); dfree(lp->d_gc); dfree(lp->d_gcp); dfree(lp->d_prop); dfree(lp->d_dbg);
     dfree(lp->d_part); dfree(lp->d_mctr); dfree(lp->d_lacc); dfree(lp->d_lprop); dfree(lp->d_terms);
@@ -122,22 +122,25 @@ int grad_splits(int M) {
     return sp;
 }
 
+// The gradient of a backward pass over M rows: with split K it is left as
+// grad_splits(M) slices of the flat parameter vector in d_gpart (slice z at
+// d_gpart + z n_par), to be summed in slice order; otherwise it is d_g.
+const float* grad_slices(const pspin_learned* lp, int M) { return grad_splits(M) > 1 ? lp->d_gpart : lp->d_g; }
+
 // dW [rows][cols] = dZ^T X and db [rows] = column sums of dZ, dZ [M][rows]
-// and X [M][cols] BF16, into the flat gradient at (off_w, off_b).
+// and X [M][cols] BF16, at (off_w, off_b) of the gradient's slices.
 hipError_t weight_grad(pspin_learned* lp, const uint16_t* dZ, int rows, const uint16_t* X, int cols, int M,
                        long long off_w, long long off_b) {
     const hipStream_t s = lp->st->stream;
     const int sp = grad_splits(M);
+    float* G = (float*)grad_slices(lp, M);
     ps::GemmArgs g{};
     g.A = dZ; g.a_m = 1; g.a_k = rows;
     g.B = X; g.b_n = 1; g.b_k = cols;
     g.M = rows; g.N = cols; g.K = M;
-    g.Cf = sp > 1 ? lp->d_gpart : lp->d_g + off_w; g.ldc = cols;
-    g.rowsum = sp > 1 ? lp->d_bpart : lp->d_g + off_b;
-    hipError_t e = ps::launch_gemm(g, ps::kEpiF32, sp, s);
-    if (e || sp == 1) return e;
-    if ((e = ps::launch_sum_slices(lp->d_gpart, sp, (long long)rows * cols, lp->d_g + off_w, s))) return e;
-    return ps::launch_sum_slices(lp->d_bpart, sp, rows, lp->d_g + off_b, s);
+    g.Cf = G + off_w; g.ldc = cols; g.cz = lp->n_par;
+    g.rowsum = G + off_b; g.rz = lp->n_par;
+    return ps::launch_gemm(g, ps::kEpiF32, sp, s);
 }
 
 // dX = (dZ W) * silu'(Z) into BF16 out, dZ [M][n_out], W [n_out][n_in].
@@ -382,10 +385,7 @@ int pspin_lp_create(pspin_state* st, const pspin_lp_config* cfg, pspin_learned**
     if (!e) e = dmalloc(&lp->d_v, np);
     if (!e) e = dmalloc(&lp->d_g, np);
     if (!e) e = dmalloc(&lp->d_pb, np);
-    const long long wmax = std::max({(long long)lp->out_dim * lp->hid, (long long)lp->hid * lp->hid,
-                                     (long long)lp->hid * lp->in_pad});
-    if (!e) e = dmalloc(&lp->d_gpart, (size_t)kMaxSplits * wmax);
-    if (!e) e = dmalloc(&lp->d_bpart, (size_t)kMaxSplits * std::max(lp->out_dim, lp->hid));
+    if (!e) e = dmalloc(&lp->d_gpart, (size_t)kMaxSplits * np);
     if (!e) e = alloc_act(lp->tr, cfg->batch_cap, lp, true);
     if (!e) e = alloc_act(lp->fw, lp->Rpad, lp, false);
     if (!e) e = alloc_act(lp->rv, lp->Rpad, lp, false);
@@ -484,6 +484,7 @@ int pspin_lp_loss_grad(pspin_learned* lp, const float* x, const float* u, const 
     LP_TRY_HIP(ps::launch_lp_nll(lp->tr.O, lp->out_dim, lp->tr.U, lp->tr.Y, lp->dim, Mpad, M, lp->cfg.log_sigma0,
                                  lp->cfg.log_sigma_min, lp->cfg.log_sigma_max, lp->tr.dO, lp->tr.rowloss, s));
     LP_TRY_HIP(backward(lp, lp->tr, Mpad));
+    if (grad_splits(Mpad) > 1) LP_TRY_HIP(ps::launch_sum_slices(lp->d_gpart, grad_splits(Mpad), lp->n_par, lp->d_g, s));
     std::vector<double> rl((size_t)M);
     LP_TRY_HIP(hipMemcpyAsync(rl.data(), lp->tr.rowloss, (size_t)M * 8, hipMemcpyDeviceToHost, s));
     if (grad) LP_TRY_HIP(hipMemcpyAsync(grad, lp->d_g, (size_t)lp->n_par * 4, hipMemcpyDeviceToHost, s));
@@ -540,8 +541,8 @@ int pspin_lp_train(pspin_learned* lp, long long n_steps, int batch, int lag, dou
                                   lp->cfg.log_sigma0, lp->cfg.log_sigma_min, lp->cfg.log_sigma_max, lp->tr.dO,
                                   lp->tr.rowloss, s);
         if (!e) e = backward(lp, lp->tr, batch);
-        if (!e) e = ps::launch_adam(lp->d_p, lp->d_m, lp->d_v, lp->d_g, lp->d_pb, lp->n_par, (float)lr, 0.9f, 0.999f,
-                                    1e-8f, ++lp->adam_t, s);
+        if (!e) e = ps::launch_adam(lp->d_p, lp->d_m, lp->d_v, grad_slices(lp, batch), grad_splits(batch), lp->d_pb,
+                                    lp->n_par, (float)lr, 0.9f, 0.999f, 1e-8f, ++lp->adam_t, s);
         if (!e) e = ps::launch_sum_rows(lp->tr.rowloss, batch, 1.0 / batch, d_hist + it, s);
         if (e) {
             ps::set_error(std::string("HIP error in pspin_lp_train: ") + hipGetErrorString(e));

@@ -50,14 +50,14 @@ __device__ __forceinline__ void fetch(const uint16_t* __restrict__ P, long long 
     v1 = *(const uint4*)(q + 8);
 }
 
-__device__ __forceinline__ uint16_t half_of(const uint4& v, int i) {
+__device__ __forceinline__ uint16_t half_of(uint4 v, int i) {
     const uint32_t w = i < 2 ? v.x : (i < 4 ? v.y : (i < 6 ? v.z : v.w));
     return (uint16_t)((i & 1) ? (w >> 16) : (w & 0xFFFFu));
 }
 
 // Registers -> LDS tile L[row][k].
 template <bool KC>
-__device__ __forceinline__ void put(const uint4& v0, const uint4& v1, uint16_t (*L)[kPitch]) {
+__device__ __forceinline__ void put(uint4 v0, uint4 v1, uint16_t (*L)[kPitch]) {
     const int t = threadIdx.x;
     if (KC) {
         const int r = t >> 2, c = (t & 3) * 16;
@@ -148,8 +148,9 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs a) {
         __syncthreads();
         cur ^= 1;
     }
-    if (rows) a.rowsum[(long long)blockIdx.z * a.M + m0 + threadIdx.x] = rs;
-    epilogue<EPI>(a, acc, a.Cf + (long long)blockIdx.z * a.M * a.ldc, m0 + wm, n0 + wn + fr, lane);
+    if (rows) a.rowsum[(long long)blockIdx.z * (a.rz ? a.rz : a.M) + m0 + threadIdx.x] = rs;
+    epilogue<EPI>(a, acc, a.Cf + (long long)blockIdx.z * (a.cz ? a.cz : (long long)a.M * a.ldc), m0 + wm,
+                  n0 + wn + fr, lane);
 }
 
 // Forward product of a few rows (the learned move's M = R rows): one wave
@@ -266,11 +267,12 @@ hipError_t launch_sum_slices(const float* part, int nz, long long n, float* out,
 
 // Adam (Kingma & Ba 2015) on FP32 masters; refreshes the BF16 copy.
 __global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
-                       const float* __restrict__ g, uint16_t* __restrict__ pb, long long n, float lr, float b1,
-                       float b2, float eps, float bc1, float bc2) {
+                       const float* __restrict__ g, int nz, uint16_t* __restrict__ pb, long long n, float lr,
+                       float b1, float b2, float eps, float bc1, float bc2) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float gi = g[i];
+    float gi = 0.0f;  // split-K slices in order, as k_sum_slices adds them
+    for (int z = 0; z < nz; z++) gi += g[(long long)z * n + i];
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
@@ -280,11 +282,11 @@ __global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __re
     pb[i] = f2bf(pi);
 }
 
-hipError_t launch_adam(float* p, float* m, float* v, const float* g, uint16_t* pb, long long n, float lr,
+hipError_t launch_adam(float* p, float* m, float* v, const float* g, int nz, uint16_t* pb, long long n, float lr,
                        float b1, float b2, float eps, long long t, hipStream_t s) {
     const float bc1 = 1.0f - powf(b1, (float)t), bc2 = 1.0f - powf(b2, (float)t);
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, m, v, g, pb, n, lr, b1, b2,
-                       eps, bc1, bc2);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, m, v, g, nz, pb, n, lr, b1,
+                       b2, eps, bc1, bc2);
     return hipGetLastError();
 }
 

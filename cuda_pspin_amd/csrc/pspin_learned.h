@@ -38,13 +38,17 @@ struct GemmArgs {
     const float* Z;     // pre-activation read by kEpiDSilu
     long long ldz;
     float* rowsum;      // [splits][M] row sums of A over each K slice, or null
+    long long cz;       // split-K slice strides of Cf and rowsum (0: M * ldc and M)
+    long long rz;
 };
 
 // splits > 1 (kEpiF32 only): Cf receives `splits` partial products, slice z
-// at Cf + z * M * ldc, to be summed by launch_sum_slices.
+// at Cf + z * cz, to be summed in slice order (launch_sum_slices, launch_adam).
 hipError_t launch_gemm(const GemmArgs& g, int epi, int splits, hipStream_t s);
 hipError_t launch_sum_slices(const float* part, int nz, long long n, float* out, hipStream_t s);
-hipError_t launch_adam(float* p, float* m, float* v, const float* g, uint16_t* pb, long long n, float lr,
+// Adam on FP32 masters with the gradient g[i] = sum over z < nz of g[z n + i]
+// (split-K slices summed in order, as launch_sum_slices does).
+hipError_t launch_adam(float* p, float* m, float* v, const float* g, int nz, uint16_t* pb, long long n, float lr,
                        float b1, float b2, float eps, long long t, hipStream_t s);
 hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, long long n, hipStream_t s);
 
@@ -129,8 +133,7 @@ struct pspin_learned {
     float* d_g = nullptr;
     uint16_t* d_pb = nullptr;
     long long adam_t = 0;
-    float* d_gpart = nullptr;  // split-K partials of weight gradients [kSplits][max layer size]
-    float* d_bpart = nullptr;  // split-K partials of bias gradients [kSplits][max width]
+    float* d_gpart = nullptr;  // split-K slices of the gradient [kSplits][n_par]
 
     pspin_mlp_act tr;  // training batch
     pspin_mlp_act fw;  // move: forward pass at the current state

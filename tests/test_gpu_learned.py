@@ -37,16 +37,29 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, float) - np.asarray(b, float)) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_mlp_forward_matches_oracle(pspin):
-    st = _state(pspin)
-    lp = _lp(pspin, st)
+# (N, hidden, rows): a small network, and the benchmark's (N = 64, hidden 512)
+# at its training batch of 4096 rows (K = 512 products, weight gradients in 8
+# K slices of 512 rows).
+SHAPES = [(16, 128, 200), (64, 512, 4096)]
+
+
+def _lp_shape(pspin, st, hidden, rows):
+    if rows <= 512:
+        return _lp(pspin, st, hidden=hidden)
+    return pspin.LearnedMove(st, pspin.LPConfig.default(hidden=hidden, batch_cap=rows, data_cap=4))
+
+
+@pytest.mark.parametrize("N,hidden,rows", SHAPES)
+def test_mlp_forward_matches_oracle(pspin, N, hidden, rows):
+    st = _state(pspin, N=N)
+    lp = _lp_shape(pspin, st, hidden, rows)
     d = mlp.Dims(N=st.N, hid=lp.hidden)
     assert lp.n_params == d.n_params and lp.in_pad == d.in_pad
     rng = np.random.default_rng(1)
     p = mlp.init_params(d, rng, gain=1.0)
     lp.set_params(p)
     assert np.array_equal(lp.params(), p)
-    x = rng.normal(size=(200, d.in_dim)).astype(np.float32)
+    x = rng.normal(size=(rows, d.in_dim)).astype(np.float32)
     got = lp.forward(x)
     want, _ = mlp.forward(p, d, x, emulate=True)
     plain, _ = mlp.forward(p, d, x, emulate=False)
@@ -59,14 +72,15 @@ def test_mlp_forward_matches_oracle(pspin):
     st.close()
 
 
-def test_mlp_loss_gradient_matches_oracle(pspin):
-    st = _state(pspin)
-    lp = _lp(pspin, st)
+@pytest.mark.parametrize("N,hidden,rows", SHAPES)
+def test_mlp_loss_gradient_matches_oracle(pspin, N, hidden, rows):
+    st = _state(pspin, N=N)
+    lp = _lp_shape(pspin, st, hidden, rows)
     d = mlp.Dims(N=st.N, hid=lp.hidden)
     rng = np.random.default_rng(2)
     p = mlp.init_params(d, rng, gain=0.5)
     lp.set_params(p)
-    M = 200
+    M = rows
     x = rng.normal(size=(M, d.in_dim)).astype(np.float32)
     U = (x[:, :d.dim] + 0.05 * rng.normal(size=(M, d.dim))).astype(np.float32)
     Y = (U + 0.1 * rng.normal(size=(M, d.dim))).astype(np.float32)
