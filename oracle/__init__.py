@@ -93,6 +93,9 @@ def lib() -> C.CDLL:
             "or_cpu_pt_lists": (None, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _u8p, _dp, _u64p, C.c_int,
                                        C.c_long, C.c_int, _dp, _dp, _dp, _i64p, _i64p, _i64p, _i64p,
                                        C.POINTER(C.c_int), C.c_int, _dp]),
+            "or_cpu_pt_sparse": (None, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _i32p, _dp, _dp,
+                                        _u64p, C.c_int, C.c_long, C.c_int, _dp, _dp, _dp, _i64p, _i64p, _i64p,
+                                        _i64p, C.POINTER(C.c_int), C.c_int, _dp]),
             "or_log_jn": (C.c_double, [C.c_int, C.c_double]),
             "or_lp_log_density": (C.c_double, [C.c_int, _dp, _dp, _dp]),
             "or_lp_gauge": (C.c_int, [C.c_int, _dp, _dp, _dp]),
@@ -325,6 +328,27 @@ def cpu_pt(N, NT, nr, spins, g2, h2_active, g4, mask, beta_sched, rseeds, n_swee
                     np.ascontiguousarray(beta_sched, dtype=np.float64), np.ascontiguousarray(rseeds, dtype=np.uint64),
                     n_sweeps, seed48, measure_from, out["e"], out["e2"], out["q2"], out["acc"], out["prop"],
                     out["ex_acc"], out["ex_prop"], C.byref(n), int(qbins), out["qh"])
+    out["n"] = n.value
+    out["spins"] = sp
+    return out
+
+
+def cpu_pt_sparse(N, NT, nr, spins, g2, h2_active, quartets, gq, beta_sched, rseeds, n_sweeps, seed48, measure_from,
+                  qbins=0):
+    """cpu_pt for a sparse (smoothed-cube) system: quartets [nq][5] (i, j, k, l, ch), couplings gq [nq]
+    (src/mc_sparse.cu:63-253 chains in the reference's PT loop)."""
+    sp = _spins(spins).copy()
+    q = np.ascontiguousarray(quartets, dtype=np.int32).reshape(-1, 5)
+    out = dict(e=np.zeros(NT), e2=np.zeros(NT), q2=np.zeros(NT), acc=np.zeros(NT, np.int64),
+               prop=np.zeros(NT, np.int64), ex_acc=np.zeros(NT - 1, np.int64), ex_prop=np.zeros(NT - 1, np.int64),
+               qh=np.zeros((NT, max(qbins, 1))))
+    n = C.c_int()
+    lib().or_cpu_pt_sparse(N, NT, nr, sp, np.ascontiguousarray(g2, dtype=np.float64).reshape(-1), int(h2_active),
+                           len(q), q.reshape(-1), np.ascontiguousarray(gq, dtype=np.float64),
+                           np.ascontiguousarray(beta_sched, dtype=np.float64),
+                           np.ascontiguousarray(rseeds, dtype=np.uint64), n_sweeps, seed48, measure_from, out["e"],
+                           out["e2"], out["q2"], out["acc"], out["prop"], out["ex_acc"], out["ex_prop"], C.byref(n),
+                           int(qbins), out["qh"])
     out["n"] = n.value
     out["spins"] = sp
     return out

@@ -765,6 +765,17 @@ static void lists_sweep_one(int N, double* s, const double* g2, int h2_active, c
     }
 }
 
+// The PT driver loop shared by the dense and sparse CPU chains: sweep(r,
+// beta, sweep index, &E[r], &acc) advances replica r by one sweep; E holds
+// the replicas' initial energies.
+extern "C++" {
+template <typename Sweep>
+static void cpu_pt_loop(Sweep sweep, std::vector<double>& E, int N, int NT, int nr, double* spins,
+                        const double* beta_sched, int n_sweeps, long seed48, int measure_from, double* e_sum,
+                        double* e2_sum, double* q2_sum, long long* acc, long long* prop, long long* ex_acc,
+                        long long* ex_prop, int* n_meas, int qbins, double* qh_sum);
+}
+
 static void cpu_pt_impl(bool lists, int N, int NT, int nr, double* spins, const double* g2, int h2_active,
                         const double* g4, const uint8_t* mask, const double* beta_sched, const uint64_t* rseeds,
                         int n_sweeps, long seed48, int measure_from, double* e_sum, double* e2_sum, double* q2_sum,
@@ -796,14 +807,32 @@ static void cpu_pt_impl(bool lists, int N, int NT, int nr, double* spins, const 
     const int R = NT * nr;
     std::unique_ptr<SiteTerms> L;
     if (lists) L.reset(new SiteTerms(N, g4, mask));
-    std::vector<int> perm(R), slot(R);
-    for (int i = 0; i < R; i++) perm[i] = i;
     std::vector<double> E(R);
     for (int r = 0; r < R; r++) {
         double e2, e4;
         or_energy(N, spins + (size_t)2 * N * r, g2, h2_active, g4, mask, &e2, &e4);
         E[r] = e2 + e4;
     }
+    auto sweep = [&](int r, double beta, int sw, double* Er, long long* a) {
+        if (lists)
+            lists_sweep_one(N, spins + (size_t)2 * N * r, g2, h2_active, *L, beta, rseeds[r], (uint64_t)sw, Er, a);
+        else
+            ref_sweep_one(N, spins + (size_t)2 * N * r, g2, h2_active, g4, mask, beta, rseeds[r], (uint64_t)sw, Er,
+                          a);
+    };
+    cpu_pt_loop(sweep, E, N, NT, nr, spins, beta_sched, n_sweeps, seed48, measure_from, e_sum, e2_sum, q2_sum, acc,
+                prop, ex_acc, ex_prop, n_meas, qbins, qh_sum);
+}
+
+extern "C++" {
+template <typename Sweep>
+static void cpu_pt_loop(Sweep sweep, std::vector<double>& E, int N, int NT, int nr, double* spins,
+                        const double* beta_sched, int n_sweeps, long seed48, int measure_from, double* e_sum,
+                        double* e2_sum, double* q2_sum, long long* acc, long long* prop, long long* ex_acc,
+                        long long* ex_prop, int* n_meas, int qbins, double* qh_sum) {
+    const int R = NT * nr;
+    std::vector<int> perm(R), slot(R);
+    for (int i = 0; i < R; i++) perm[i] = i;
     uint64_t X = (((uint64_t)(uint32_t)seed48) << 16) | 0x330EULL;
     *n_meas = 0;
     for (int sw = 0; sw < n_sweeps; sw++) {
@@ -812,12 +841,7 @@ static void cpu_pt_impl(bool lists, int N, int NT, int nr, double* spins, const 
         for (int r = 0; r < R; r++) {
             long long a = 0;
             const int t = slot[r];
-            if (lists)
-                lists_sweep_one(N, spins + (size_t)2 * N * r, g2, h2_active, *L, beta_sched[t], rseeds[r],
-                                (uint64_t)sw, &E[r], &a);
-            else
-                ref_sweep_one(N, spins + (size_t)2 * N * r, g2, h2_active, g4, mask, beta_sched[t], rseeds[r],
-                              (uint64_t)sw, &E[r], &a);
+            sweep(r, beta_sched[t], sw, &E[r], &a);
             if (sw >= measure_from) {
                 acc[t] += a;
                 prop[t] += N / 2;
@@ -861,6 +885,100 @@ static void cpu_pt_impl(bool lists, int N, int NT, int nr, double* spins, const 
             (*n_meas)++;
         }
     }
+}
+}  // extern "C++"
+
+// ---------------------------------------------------------------------------
+// Sparse (smoothed-cube) chains (src/mc_sparse.cu:63-253) in the same PT
+// driver loop (src/parallel_tempering_sparse.cu runs the dense driver's
+// exchange): the cube move (include/spins_sparse.h:26-56) from the step's
+// uniforms, H2 as in the dense sweep, H4 over the selected quartets holding
+// i0 or j0 (i0's list, then j0's without those holding i0) as g (f_new -
+// f_old) in FP64 -- the reference's full and differential products, equal
+// up to rounding -- and Metropolis with exp.
+// ---------------------------------------------------------------------------
+struct SparseTerms {
+    const int* q;
+    const double* g;
+    std::vector<std::vector<int>> of_site;
+    SparseTerms(int N, int nq, const int* quartets, const double* gq) : q(quartets), g(gq), of_site(N) {
+        for (int t = 0; t < nq; t++)
+            for (int p = 0; p < 4; p++) {
+                std::vector<int>& v = of_site[quartets[5 * t + p]];
+                if (v.empty() || v.back() != t) v.push_back(t);
+            }
+    }
+};
+
+static void sparse_sweep_one(int N, double* s, const double* g2, int h2_active, const SparseTerms& L, double beta,
+                             uint64_t rseed, uint64_t sweep, double* E, long long* acc) {
+    for (int step = 0; step < N / 2; step++) {
+        double u[6];
+        or_step_uniforms(rseed, sweep, step, u);
+        int i0, j0;
+        or_pair_pick(N, u[0], u[1], &i0, &j0);
+        double ai[2] = {s[2 * i0], s[2 * i0 + 1]}, aj[2] = {s[2 * j0], s[2 * j0 + 1]};
+        double ni[2], nj[2];
+        or_propose_cube(ai, aj, u[2], u[3], u[4], ni, nj);
+        cplx di = {ni[0] - ai[0], ni[1] - ai[1]}, dj = {nj[0] - aj[0], nj[1] - aj[1]};
+        double dE = 0.0;
+        if (h2_active) {
+            cplx si = {0, 0}, sj = {0, 0};
+            for (int k = 0; k < N; k++) {
+                if (k == i0 || k == j0) continue;
+                cplx akc = cconj(ld(s, k));
+                si.re += g2[i0 * N + k] * akc.re; si.im += g2[i0 * N + k] * akc.im;
+                sj.re += g2[j0 * N + k] * akc.re; sj.im += g2[j0 * N + k] * akc.im;
+            }
+            dE -= cmul(di, si).re + cmul(dj, sj).re;
+            double gij = g2[i0 * N + j0];
+            cplx op = cmul({ai[0], ai[1]}, cconj({aj[0], aj[1]}));
+            cplx np = cmul({ni[0], ni[1]}, cconj({nj[0], nj[1]}));
+            dE -= gij * (np.re - op.re);
+        }
+        for (int pass = 0; pass < 2; pass++)
+            for (int t : L.of_site[pass == 0 ? i0 : j0]) {
+                const int* v = L.q + 5 * t;
+                if (pass == 1 && (v[0] == i0 || v[1] == i0 || v[2] == i0 || v[3] == i0)) continue;
+                int idx[4] = {v[0], v[1], v[2], v[3]};
+                cplx fo = quartet_product(s, idx, v[4]);
+                double sv[8];
+                for (int p = 0; p < 4; p++) {
+                    int x = idx[p];
+                    const double* src = (x == i0) ? ni : (x == j0) ? nj : &s[2 * x];
+                    sv[2 * p] = src[0]; sv[2 * p + 1] = src[1];
+                }
+                int loc[4] = {0, 1, 2, 3};
+                cplx fn = quartet_product(sv, loc, v[4]);
+                dE -= L.g[t] * (fn.re - fo.re);
+            }
+        if (or_metropolis(beta, dE, u[5])) {
+            s[2 * i0] = ni[0]; s[2 * i0 + 1] = ni[1];
+            s[2 * j0] = nj[0]; s[2 * j0 + 1] = nj[1];
+            *E += dE;
+            (*acc)++;
+        }
+    }
+}
+
+// CPU PT of a sparse system: the measurements of or_cpu_pt.
+void or_cpu_pt_sparse(int N, int NT, int nr, double* spins, const double* g2, int h2_active, int nq,
+                      const int* quartets, const double* gq, const double* beta_sched, const uint64_t* rseeds,
+                      int n_sweeps, long seed48, int measure_from, double* e_sum, double* e2_sum, double* q2_sum,
+                      long long* acc, long long* prop, long long* ex_acc, long long* ex_prop, int* n_meas, int qbins,
+                      double* qh_sum) {
+    const int R = NT * nr;
+    const SparseTerms L(N, nq, quartets, gq);
+    std::vector<double> E(R);
+    for (int r = 0; r < R; r++) {
+        const double* sr = spins + (size_t)2 * N * r;
+        E[r] = (h2_active ? or_energy_h2(N, sr, g2) : 0.0) + or_energy_h4_sparse(sr, nq, quartets, gq);
+    }
+    auto sweep = [&](int r, double beta, int sw, double* Er, long long* a) {
+        sparse_sweep_one(N, spins + (size_t)2 * N * r, g2, h2_active, L, beta, rseeds[r], (uint64_t)sw, Er, a);
+    };
+    cpu_pt_loop(sweep, E, N, NT, nr, spins, beta_sched, n_sweeps, seed48, measure_from, e_sum, e2_sum, q2_sum, acc,
+                prop, ex_acc, ex_prop, n_meas, qbins, qh_sum);
 }
 
 // ---------------------------------------------------------------------------

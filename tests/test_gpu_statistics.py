@@ -127,3 +127,106 @@ def test_gpu_pt_matches_reference_logic_statistically(pspin, oracle, system):
         print(f"  {nm:14s} gpu {g: .5f}  cpu {c: .5f}  z {zz: .2f}")
     assert np.all(np.abs(z) < 3.5), z
     assert np.mean(np.abs(z) > 2.0) <= 0.25, z
+
+
+# Sparse (smoothed-cube) systems: N = 24 with N quartets sampled from the
+# uniform-FMC survivors (gamma = 0.2), the single-state kernel (LISTS layout,
+# cube move) and the batched kernel (all 12 samples in one launch), against
+# the oracle's CPU PT of the same sparse systems (src/mc_sparse.cu:63-253
+# chains in the reference's PT loop).
+SPARSE_N, SPARSE_GAMMA = 24, 0.2
+
+
+def _sparse_state(pspin, seed, beta):
+    cfg = pspin.SimConfig.default(N=SPARSE_N, nrep=NT * NR, seed=seed, fmc_mode=2, gamma=SPARSE_GAMMA)
+    st = pspin.PSpin(cfg, pspin.SPARSE)
+    st.pt_init(NT, NR, np.asarray(beta), seed + 5000)
+    return st
+
+
+def _observe(N, E, perm, z, acc):
+    """One measurement of a PT state: per slot E/N, (E/N)^2, |q|^2 and the |q| bin."""
+    perm = perm.reshape(NT, NR)
+    e = (E[perm] / N).mean(axis=1)
+    e2 = ((E[perm] / N) ** 2).mean(axis=1)
+    q = np.sum(z[perm[:, 0]] * np.conj(z[perm[:, 1]]), axis=1) / N
+    acc[0] += e
+    acc[1] += e2
+    acc[2] += np.abs(q) ** 2
+    acc[3][np.arange(NT), np.minimum(QB - 1, (np.abs(q) * QB).astype(int))] += 1
+
+
+def _finish(N, beta, acc, n, a, p, xa, xp):
+    e, e2, q2, qh = acc
+    cv = N * beta ** 2 * (e2 / n - (e / n) ** 2)
+    return np.concatenate([e / n, cv, q2 / n, (qh / n).reshape(-1), [a.sum() / p.sum()], xa / xp])
+
+
+@pytest.mark.parametrize("kernel", ["single", "batch"])
+def test_gpu_sparse_pt_matches_reference_logic_statistically(pspin, oracle, kernel):
+    print(f"[statistics] sparse cube N={SPARSE_N} {kernel}", flush=True)
+    N = SPARSE_N
+    T, beta = oracle.temperatures(NT, 0.5, 2.0, 1)
+    seeds = [2000 + 11 * s for s in range(12)]
+    data, gpu = [], []
+    for sd in seeds:  # disorder, spins and RNG seeds of each sample (the batch builds the same)
+        st = _sparse_state(pspin, sd, beta)
+        q, gq = st.sparse_quartets()
+        rs, _ = st.rng_info()
+        data.append(dict(s0=st.spins(), g2=st.g2(), q=q, gq=gq, rs=rs, h2=st.info.h2_active))
+        if kernel == "single":
+            st.pt_run(BURN, 1)
+            st.reset_counters()
+            st.pt_reset_window()
+            acc = [np.zeros(NT), np.zeros(NT), np.zeros(NT), np.zeros((NT, QB))]
+            for _ in range(MEAS // 2):
+                st.pt_run(2, 1)
+                _observe(N, st.results()[0], st.pt_perm(), st.spins(), acc)
+            _, a, p = st.results()
+            xa, xp, _, _ = st.pt_exchange_stats()
+            gpu.append(_finish(N, beta, acc, MEAS // 2, a, p, xa, xp))
+        st.close()
+    if kernel == "batch":
+        base = pspin.SimConfig.default(fmc_mode=2, gamma=SPARSE_GAMMA)
+        b = pspin.PSpinBatch(base, [N] * len(seeds), seeds, NT, NR, beta)
+        b.set_resync_interval(0)
+        b.run(BURN, 1)
+        a0 = [b.results(s)[1:] for s in range(len(seeds))]
+        x0 = [b.exchange_stats(s) for s in range(len(seeds))]
+        accs = [[np.zeros(NT), np.zeros(NT), np.zeros(NT), np.zeros((NT, QB))] for _ in seeds]
+        for _ in range(MEAS // 2):
+            b.run(2, 1)
+            for s in range(len(seeds)):
+                _observe(N, b.results(s)[0], b.perm(s), b.spins(s), accs[s])
+        for s in range(len(seeds)):
+            _, a, p = b.results(s)
+            xa, xp = b.exchange_stats(s)
+            gpu.append(_finish(N, beta, accs[s], MEAS // 2, a - a0[s][0], p - a0[s][1], xa - x0[s][0],
+                               xp - x0[s][1]))
+        b.close()
+
+    def cpu_run(args):
+        sd, d = args
+        rs = d["rs"] ^ np.uint64(0x9E3779B97F4A7C15)
+        out = oracle.cpu_pt_sparse(N, NT, NR, d["s0"], d["g2"], d["h2"], d["q"], d["gq"], beta, rs, BURN + MEAS,
+                                   sd + 5000 + 17, BURN, qbins=QB)
+        n = out["n"]
+        e = out["e"] / (n * NR)
+        cv = N * beta ** 2 * (out["e2"] / (n * NR) - e ** 2)
+        return np.concatenate([e, cv, out["q2"] / n, (out["qh"] / n).reshape(-1),
+                               [out["acc"].sum() / out["prop"].sum()], out["ex_acc"] / out["ex_prop"]])
+
+    with cf.ThreadPoolExecutor(max_workers=12) as ex:
+        cpu = list(ex.map(cpu_run, zip(seeds, data)))
+    gpu, cpu = np.array(gpu), np.array(cpu)
+    diff = gpu - cpu
+    S = len(seeds)
+    jk = np.array([np.delete(diff, i, axis=0).mean(axis=0) for i in range(S)])
+    se = np.sqrt((S - 1) / S * np.sum((jk - jk.mean(axis=0)) ** 2, axis=0))
+    z = diff.mean(axis=0) / np.maximum(se, 1e-12)
+    names = ([f"e[T={t:.3f}]" for t in T] + [f"Cv[T={t:.3f}]" for t in T] + [f"q2[T={t:.3f}]" for t in T] +
+             [f"P|q|{b}[T={t:.3f}]" for t in T for b in range(QB)] + ["acc"] + [f"ex[{t}]" for t in range(NT - 1)])
+    for nm, g, c, zz in zip(names, gpu.mean(axis=0), cpu.mean(axis=0), z):
+        print(f"  {nm:14s} gpu {g: .5f}  cpu {c: .5f}  z {zz: .2f}")
+    assert np.all(np.abs(z) < 3.5), z
+    assert np.mean(np.abs(z) > 2.0) <= 0.25, z

@@ -342,3 +342,78 @@ def test_sparse_select_hand_trace(oracle):
     assert q.tolist() == [[1, 2, 3, 4, 1], [0, 1, 3, 4, 1]]
     s, m = J4 * math.sqrt(5 / 2), J40 * 5 / 2
     assert g.tolist() == [3.0 * s + m, -2.0 * s + m]
+
+
+def _sparse_system(rng, N, nq):
+    """N sites, nq random quartets (distinct sites, channel 0..2), Gaussian
+    couplings, symmetric g2, replicas on the cube sum |a|^4 = N."""
+    q = np.zeros((nq, 5), dtype=np.int32)
+    for t in range(nq):
+        q[t, :4] = np.sort(rng.choice(N, 4, replace=False))
+        q[t, 4] = rng.integers(0, 3)
+    gq = rng.normal(size=nq)
+    g2 = rng.normal(size=(N, N))
+    g2 = (g2 + g2.T) / 2
+    np.fill_diagonal(g2, 0.0)
+    return q, gq, g2
+
+
+def _cube_spins(rng, R, N):
+    a = rng.normal(size=(R, N)) + 1j * rng.normal(size=(R, N))
+    a *= (N / np.sum(np.abs(a) ** 4, axis=1, keepdims=True)) ** 0.25
+    return a
+
+
+def test_cpu_pt_sparse_replays_step_by_step(oracle):
+    """The sparse CPU chain is the composition of the pinned primitives:
+    step uniforms, pair pick (src/mc.cu:129-141), cube move
+    (include/spins_sparse.h:26-56), sparse Delta E (src/mc_sparse.cu:146-210)
+    and the Metropolis rule, in that order, every step.  A replay in Python
+    from those primitives must end on the same spins and counters (no
+    exchange: one temperature)."""
+    rng = np.random.default_rng(4)
+    N, nr, sweeps = 10, 3, 6
+    q, gq, g2 = _sparse_system(rng, N, N)
+    a0 = _cube_spins(rng, nr, N)
+    beta = np.array([1.3])
+    seeds = oracle.replica_seeds(19, nr)
+    out = oracle.cpu_pt_sparse(N, 1, nr, a0, g2, 1, q, gq, beta, seeds, sweeps, 3, 0)
+    got = out["spins"].reshape(nr, N, 2)
+    acc = 0
+    for r in range(nr):
+        s = a0[r].copy()
+        for sw in range(sweeps):
+            for k in range(N // 2):
+                u = oracle.step_uniforms(int(seeds[r]), sw, k)
+                i0, j0 = oracle.pair_pick(N, u[0], u[1])
+                ni, nj = oracle.propose_cube(s[i0], s[j0], u[2], u[3], u[4])
+                ni, nj = complex(*ni), complex(*nj)
+                dE, _ = oracle.delta_e_sparse(N, s, g2, 1, q, gq, i0, j0, ni, nj)
+                if dE <= 0 or u[5] < math.exp(-beta[0] * dE):
+                    s[i0], s[j0] = ni, nj
+                    acc += 1
+        np.testing.assert_allclose(got[r, :, 0] + 1j * got[r, :, 1], s, rtol=0, atol=1e-12)
+    assert out["acc"].sum() == acc and 0 < acc < nr * sweeps * (N // 2)
+    assert out["prop"].sum() == nr * sweeps * (N // 2)
+
+
+def test_cpu_pt_sparse_energy_and_constraint(oracle):
+    """Over a PT run the sparse chains keep sum |a|^4 = N, and the energies
+    they carry (sums of per-step Delta E, reported per slot) equal the FP64
+    energies of the final spins: a dropped, doubled or sign-flipped term of
+    Delta E breaks the second.  At beta = 0 every move is accepted."""
+    rng = np.random.default_rng(8)
+    N, NT, nr, sweeps = 14, 3, 2, 300
+    q, gq, g2 = _sparse_system(rng, N, N)
+    a0 = _cube_spins(rng, NT * nr, N)
+    beta = np.array([0.4, 1.0, 2.5])
+    out = oracle.cpu_pt_sparse(N, NT, nr, a0, g2, 1, q, gq, beta, oracle.replica_seeds(5, NT * nr), sweeps, 11,
+                               sweeps - 1)
+    sp = out["spins"].reshape(NT * nr, N, 2)
+    a = sp[..., 0] + 1j * sp[..., 1]
+    np.testing.assert_allclose(np.sum(np.abs(a) ** 4, axis=1), N, rtol=1e-10)
+    e_full = sum(sum(oracle.energy_sparse(N, a[r], g2, 1, q, gq)) for r in range(NT * nr))
+    assert abs(out["e"].sum() * N - e_full) < 1e-9 * max(1.0, abs(e_full)), (out["e"].sum() * N, e_full)
+    assert out["ex_prop"].sum() == (NT - 1) * nr and out["n"] == 1
+    hot = oracle.cpu_pt_sparse(N, 1, 2, a0[:2], g2, 1, q, gq, np.array([0.0]), oracle.replica_seeds(5, 2), 20, 11, 0)
+    assert hot["acc"].sum() == hot["prop"].sum() == 2 * 20 * (N // 2)
