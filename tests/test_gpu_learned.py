@@ -109,14 +109,20 @@ def _slot_betas(st, beta_sched):
     return b
 
 
-@pytest.mark.parametrize("fmc", [0, 2])  # 2: uniform FMC, quartets in channels 0 and 1
-def test_learned_move_replays_on_oracle(pspin, oracle, fmc):
-    st = _state(pspin, N=16, nrep=64, seed=11, fmc=fmc)
-    N, R = st.N, st.nrep
-    T, beta = oracle.temperatures(8, 0.4, 2.0, 1)
-    st.pt_init(8, 8, beta, 99)
+# (N, NT, replicas per temperature, FMC mode, hidden width, moves): two small
+# systems (2: uniform FMC, quartets in channels 0 and 1) and the learned-move
+# benchmark's shape (N = 64 uniform FMC, 32 x 8 replicas, hidden 512).
+REPLAY_CASES = [(16, 8, 8, 0, 128, 3), (16, 8, 8, 2, 128, 3), (64, 32, 8, 2, 512, 2)]
+
+
+@pytest.mark.parametrize("N,NT,nr,fmc,hidden,moves", REPLAY_CASES)
+def test_learned_move_replays_on_oracle(pspin, oracle, N, NT, nr, fmc, hidden, moves):
+    st = _state(pspin, N=N, nrep=NT * nr, seed=11, fmc=fmc)
+    R = st.nrep
+    T, beta = oracle.temperatures(NT, 0.4, 2.0, 1)
+    st.pt_init(NT, nr, beta, 99)
     st.pt_run(40, 1)
-    lp = _lp(pspin, st, init_gain=0.3, log_sigma0=math.log(0.08))
+    lp = _lp(pspin, st, hidden=hidden, init_gain=0.3, log_sigma0=math.log(0.08))
     d = mlp.Dims(N=N, hid=lp.hidden)
     g2 = st.g2()
     g4, mask = st.g4()
@@ -125,7 +131,7 @@ def test_learned_move_replays_on_oracle(pspin, oracle, fmc):
     ls_scale = math.log(cfg.scale)
     p = lp.params()
     n_acc = n_close = 0
-    for mv in range(3):
+    for mv in range(moves):
         s0 = st.spins()
         b_r = _slot_betas(st, beta)
         lp.move()
@@ -160,9 +166,9 @@ def test_learned_move_replays_on_oracle(pspin, oracle, fmc):
             want_s = prop[r] if acc else s0[r]
             assert np.array_equal(s1[r], want_s)
             assert E1[r] == (dbg[r, 3] if acc else dbg[r, 2])
-    print(f"  replayed {3 * R} moves, {n_acc} accepted, {n_close} decided within 1e-6 of the threshold")
-    assert 0 < n_acc < 3 * R
-    assert np.all(lp.move_counters() == 3)
+    print(f"  replayed {moves * R} moves, {n_acc} accepted, {n_close} decided within 1e-6 of the threshold")
+    assert 0 < n_acc < moves * R
+    assert np.all(lp.move_counters() == moves)
     lp.close()
     st.close()
 
