@@ -539,14 +539,14 @@ int maybe_resync(pspin_state* st) {
 namespace ps {
 // PT sweeps and exchanges queued on the library stream, no final sync (the
 // body of pspin_pt_run; also used between learned moves).
-int pt_advance(pspin_state* st, int n_sweeps, int pt_freq) {
+int pt_advance(pspin_state* st, int n_sweeps, int pt_freq, int launch_rounds) {
     PS_REQUIRE(st && n_sweeps >= 0 && pt_freq >= 1, PSPIN_ERR_ARG, "bad argument");
     PS_REQUIRE(st->pt_NT >= 2, PSPIN_ERR_STATE, "pspin_pt_init has not been called");
     int done = 0;
     // Whole rounds (pt_freq sweeps + one exchange) go through one cooperative
     // launch or a captured graph of `rounds` rounds (see pspin_set_launch_mode);
     // nothing per sweep may be recorded, and graphs are not timed.
-    const int rounds = std::max(1, kGraphSweeps / pt_freq);
+    const int rounds = launch_rounds > 0 ? launch_rounds : std::max(1, kGraphSweeps / pt_freq);
     while (done < n_sweeps) {
         const int left = n_sweeps - done;
         const bool whole = st->pt_sweeps % pt_freq == 0 && left >= rounds * pt_freq && !st->d_log;

@@ -582,10 +582,20 @@ int pspin_lp_run(pspin_learned* lp, long long n_sweeps, int pt_freq, int move_ev
     pspin_state* st = lp->st;
     LP_REQUIRE(st->pt_NT >= 2, PSPIN_ERR_STATE, "pspin_pt_init has not been called");
     const bool can_graph = !st->prof && !st->d_log;
+    // The sweeps between two moves as one cooperative launch of their rounds
+    // (plain PT's launch path), then the move's captured graph; or, where
+    // cooperative launches are off, one graph of rounds and move together.
+    const bool coop = can_graph && st->launch_mode == PSPIN_LAUNCH_AUTO && ps::pt_rounds_ok(st);
     long long done = 0;
     while (done < n_sweeps) {
         const int chunk = (int)std::min<long long>(move_every, n_sweeps - done);
-        if (chunk == move_every && can_graph && st->pt_sweeps % pt_freq == 0) {
+        if (chunk == move_every && coop && st->pt_sweeps % pt_freq == 0) {
+            int rc = ps::pt_advance(st, chunk, pt_freq, chunk / pt_freq);
+            if (rc) return rc;
+            if ((rc = ensure_graph(lp, 1, 0))) return rc;
+            LP_TRY_HIP(hipGraphLaunch(lp->graph, st->stream));
+            st->since_resync = 0;
+        } else if (chunk == move_every && can_graph && st->pt_sweeps % pt_freq == 0) {
             int rc = ensure_graph(lp, pt_freq, move_every);
             if (rc) return rc;
             LP_TRY_HIP(hipGraphLaunch(lp->graph, st->stream));

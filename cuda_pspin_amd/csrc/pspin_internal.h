@@ -355,7 +355,9 @@ hipError_t launch_pt_rounds(pspin_state* st, int pt_freq, int rounds);
 hipError_t launch_exchange(pspin_state* st);
 
 // pspin_pt_run without the final stream synchronisation (api.cpp).
-int pt_advance(pspin_state* st, int n_sweeps, int pt_freq);
+// launch_rounds: PT rounds per cooperative launch or captured graph (0: the
+// default, about 32 sweeps).
+int pt_advance(pspin_state* st, int n_sweeps, int pt_freq, int launch_rounds = 0);
 // After the stream has drained: PSPIN_ERR_HIP if a grid-barrier wait of a
 // cooperative PT launch expired (rounds then ran without their exchange).
 int pt_check_barrier(pspin_state* st);
