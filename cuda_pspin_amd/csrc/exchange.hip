@@ -13,12 +13,13 @@
 // workgroup of a sweep launch): slots are staged in LDS once, lane r carries
 // ladder r's moving energy and replica down the levels in registers (level t
 // only swaps slots t and t+1, so slot t is final after level t and the values
-// of slot t+1 before it are the untouched ones).  Before the levels, the lanes
+// of slot t+1 before it are the untouched ones).  While staging, the threads
 // compute in parallel every draw the round can take (X_{k+1} = A_{k+1} X + C_{k+1})
 // and store logf of each; a level reads its draws by rank and compares their
-// logs with delta, so the chain from one level to the next holds one LDS
-// read and a few FP64 operations.  A draw within 3e-5 of the decision
-// boundary in log space is decided by the FP64 test on the draw itself.
+// logs with delta, so the chain from one level to the next holds a ballot,
+// one LDS read and a compare.  A draw within 3e-5 of the decision boundary in
+// log space is decided by the FP64 test on the draw itself.  The results go
+// out with every thread (exchange_commit).
 // More ladders (k_pt_exchange_wide) walk column blocks of 64 in LDS.
 #include "exchange_dev.h"
 
@@ -112,6 +113,8 @@ __global__ __launch_bounds__(64) void k_pt_exchange(ExArgs a) {
     exchange_stage(a, lds_ex, threadIdx.x, 64);
     __syncthreads();
     exchange_wave(a, lds_ex, threadIdx.x);
+    __syncthreads();
+    exchange_commit(a, lds_ex, threadIdx.x, 64, -1, nullptr);
 }
 
 ExArgs exchange_args(const pspin_state* st) {

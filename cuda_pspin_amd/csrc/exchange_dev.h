@@ -48,13 +48,14 @@ __device__ __forceinline__ void dev_store(T* p, T v) {
 }
 
 // LDS of the one-wave exchange (n = NT nrep slots): sE[n] energies by slot,
-// sEr[n] energies by replica (as loaded), sB[NT] betas, sacc[NT] accepted
-// swaps per level, sctr[4 NT] the exchange counters as loaded, sX0 the
-// 48-bit state and sser the series counter as loaded, sAB the accept bits of
-// each lane (64 levels per word, 64 words per group of 64 levels), sLU[n]
-// logf of every draw the round can take, sP[n + 1] replicas by slot (the
-// last entry takes the writes of lanes past the ladders).
-__host__ __device__ inline int exchange_ab_words(int NT) { return (NT - 1 + 63) / 64 * 64; }
+// sEr[n] energies by replica as loaded (after the chain: the energy that ends
+// in slot NT - 1 of each ladder), sB[NT] betas, sacc[NT] accepted swaps per
+// level, sctr[4 NT] the exchange counters as loaded, sX0 the 48-bit state and
+// sser the series counter as loaded, sDraws the draws the round took, sLU[n]
+// logf of every draw the round can take, sP[n] replicas by slot with a row of
+// nr entries before slot 0 and one after slot NT - 1 (the chain's writes of
+// "slot -1", its writes from lanes past the ladders, its read past the last
+// slot).
 struct ExLds {
     double* sE;
     double* sEr;
@@ -63,10 +64,11 @@ struct ExLds {
     long long* sctr;
     uint64_t* sX0;
     long long* sser;
-    unsigned long long* sAB;
+    long long* sDraws;
     float* sLU;
     int* sP;
-    __device__ ExLds(void* lds, int NT, int n) {
+    __device__ ExLds(void* lds, int NT, int nr) {
+        const int n = NT * nr;
         sE = (double*)lds;
         sEr = sE + n;
         sB = sEr + n;
@@ -74,9 +76,9 @@ struct ExLds {
         sctr = sacc + NT;
         sX0 = (uint64_t*)(sctr + 4 * NT);
         sser = (long long*)(sX0 + 1);
-        sAB = (unsigned long long*)(sser + 1);
-        sLU = (float*)(sAB + exchange_ab_words(NT));
-        sP = (int*)(sLU + n);
+        sDraws = sser + 1;
+        sLU = (float*)(sDraws + 1);
+        sP = (int*)(sLU + n) + nr;
     }
 };
 
@@ -84,49 +86,77 @@ struct ExLds {
 // reads from global memory is requested at once, in one round trip: the
 // permutation, every replica's energy (by replica, so no load waits for the
 // permutation), the betas, the exchange counters, the 48-bit state and the
-// series counter.  The energies are then put in slot order in LDS.  It
-// contains a workgroup barrier, so all nthr threads call it; the caller then
-// orders LDS (barrier or wave fence) before exchange_wave.
+// series counter.  Meanwhile the threads compute logf of every draw the round
+// can take.  The energies are then put in slot order in LDS.  It contains a
+// workgroup barrier, so all nthr threads call it; the caller then orders LDS
+// (barrier) before exchange_wave.
 __device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int nthr) {
     asm volatile("" : "+v"(tid));  // nothing thread-derived hoisted out of a caller's loop
-    const int NT = a.NT, n = NT * a.nrep;
-    ExLds L(lds, NT, n);
-    for (int i0 = tid; i0 < n; i0 += 4 * nthr) {
-        int p[4];
-        double e[4];
+    const int NT = a.NT, nr = a.nrep, n = NT * nr, nd = nr * (NT - 1), nc = 4 * (NT - 1);
+    ExLds L(lds, NT, nr);
+    // every load of the round is issued before any result is used
+    const uint64_t X0 = dev_load(a.rng48);
+    int p[4];
+    double e[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int i = i0 + k * nthr;
-            p[k] = i < n ? dev_load(a.perm + i) : 0;
-            e[k] = i < n ? dev_load(a.E + i) : 0.0;
+    for (int k = 0; k < 4; k++) {
+        const int i = tid + k * nthr;
+        p[k] = i < n ? dev_load(a.perm + i) : 0;
+        e[k] = i < n ? dev_load(a.E + i) : 0.0;
+    }
+    const double bs = tid < NT ? a.beta_sched[tid] : 0.0;
+    long long c[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) c[k] = tid + k * nthr < nc ? dev_load(a.ex + tid + k * nthr) : 0;
+    const long long ser = tid == 0 && a.series ? dev_load(a.series_n) : 0;
+    // logf of every draw the round can take, while the loads are in flight:
+    // draw k (k = 0 .. nd - 1) is X_{k+1}; thread tid jumps tid + 1 steps,
+    // then nthr steps at a time
+    if (tid < nd) {
+        uint64_t A, C, An = 1, Cn = 0;
+        lcg48_coeffs((uint64_t)tid + 1, &A, &C);
+        if (nd > nthr) lcg48_coeffs((uint64_t)nthr, &An, &Cn);
+        uint64_t x = (A * X0 + C) & kMask48;
+        for (int k = tid; k < nd; k += nthr) {
+            L.sLU[k] = logf((float)lcg48_unit(x));
+            x = (An * x + Cn) & kMask48;
         }
+    }
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (i0 + k * nthr < n) {
-                L.sP[i0 + k * nthr] = p[k];
-                L.sEr[i0 + k * nthr] = e[k];
-            }
-    }
-    for (int i = tid; i < NT; i += nthr) L.sB[i] = a.beta_sched[i];
-    for (int i = tid; i < 4 * (NT - 1); i += nthr) L.sctr[i] = dev_load(a.ex + i);
+    for (int k = 0; k < 4; k++)
+        if (tid + k * nthr < n) {
+            L.sP[tid + k * nthr] = p[k];
+            L.sEr[tid + k * nthr] = e[k];
+        }
+    if (tid < NT) L.sB[tid] = bs;
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (tid + k * nthr < nc) L.sctr[tid + k * nthr] = c[k];
     if (tid == 0) {
-        *L.sX0 = dev_load(a.rng48);
-        *L.sser = a.series ? dev_load(a.series_n) : 0;
+        *L.sX0 = X0;
+        *L.sser = ser;
     }
+    // what the first batches did not cover (small workgroups only)
+    for (int i = tid + 4 * nthr; i < n; i += nthr) {
+        L.sP[i] = dev_load(a.perm + i);
+        L.sEr[i] = dev_load(a.E + i);
+    }
+    for (int i = tid + nthr; i < NT; i += nthr) L.sB[i] = a.beta_sched[i];
+    for (int i = tid + 2 * nthr; i < nc; i += nthr) L.sctr[i] = dev_load(a.ex + i);
     __syncthreads();
     for (int i = tid; i < n; i += nthr) L.sE[i] = L.sEr[L.sP[i]];
 }
 
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// The level chain: lane r carries ladder r's moving energy and replica; a
-// level's draws are read by rank from the precomputed logs, so the chain
-// holds no permute, no 64-bit product, no exponential and no global access.
-// The result goes out after the last level.
+// The level chain (wave 0; lane r carries ladder r).  Level t's decision
+// needs the draws' ranks, so the chain from one level to the next runs through
+// the ballot of who needs a draw, the read of that draw's log and the compare.
+// Everything else is taken off that path: each level computes both candidate
+// deltas of the next one (the carried energy continues if this level swaps,
+// slot t+1's original is carried if not) and their need masks, and only picks
+// between them once this level's decision is known.  Accepted swaps are
+// counted per level by popcount; the slots, the counts, the draws taken and
+// the energy that ends in slot NT - 1 of each ladder go to LDS for
+// exchange_commit.
 __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
     // An opaque copy of the lane index: in a kernel that runs the exchange
     // between sweeps, nothing lane-derived here may be hoisted out of that loop
@@ -134,105 +164,142 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
     asm volatile("" : "+v"(lane));
     const int NT = a.NT, nr = a.nrep, n = NT * nr;
     const bool valid = lane < nr;
-    ExLds L(lds, NT, n);
+    ExLds L(lds, NT, nr);
     const uint64_t X0 = *L.sX0;
-    {
-        // draw k (k = 0 .. nr (NT - 1) - 1) is X_{k+1}: lane l jumps l + 1
-        // steps, then 64 steps at a time (lane 63's coefficients)
-        uint64_t A, C;
-        lcg48_coeffs((uint64_t)lane + 1, &A, &C);
-        const uint64_t A64 = readlane64(A, 63), C64 = readlane64(C, 63);
-        uint64_t x = (A * X0 + C) & kMask48;
-        for (int k = lane; k < nr * (NT - 1); k += 64) {
-            L.sLU[k] = logf((float)lcg48_unit(x));
-            x = (A64 * x + C64) & kMask48;
-        }
-    }
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // The level loop is kept free of divergent branches and of stores that
-    // wait on a lane mask (each costs the lone wave issue cycles on every
-    // level): lanes past the ladders read ladder nr - 1's slots, unused, and
-    // write their slot to the spare entry sP[n]; every lane reads the draw
-    // log of its rank; accepts are kept as bits and counted after the loop.
+    // The one wave that runs the chain takes issue priority over the
+    // workgroup's other waves (which draw the next sweep meanwhile).
+    __builtin_amdgcn_s_setprio(3);
+    // Lanes past the ladders read ladder nr - 1's slots, need no draw, never
+    // swap, and write their slots to the row after the last slot.  Lane sets
+    // (who needs a draw, who swaps) are kept as 64-bit masks, combined by
+    // scalar instructions.
     const int lr = valid ? lane : nr - 1;
-    int pc = L.sP[lr];          // replica carried down the column
-    double ec = L.sE[lr];       // its energy
-    double bt = L.sB[0];
-    int pn = L.sP[nr + lr];     // slot t+1 before level t
-    double en = L.sE[nr + lr];
-    double bn = L.sB[1];
-    int base = 0;  // draws taken so far
-    unsigned long long abits = 0;  // this lane's accepts, bit t % 64
-    for (int t = 0; t < NT - 1; t++) {
-        const int pb = pn;
-        const double eb = en, b1 = bn;
-        const int t2 = t + 2 < NT ? t + 2 : NT - 1;  // next level's untouched slot, off the chain
-        pn = L.sP[t2 * nr + lr];
-        en = L.sE[t2 * nr + lr];
-        bn = L.sB[t2];
-        // keep these reads issued here: the end of the level copies them into
-        // the next level's registers, and waits for them there
-        __builtin_amdgcn_sched_barrier(0);
-        const double delta = (bt - b1) * (ec - eb);
-        const bool need = valid && !(delta >= 0.0);
-        const unsigned long long m = __ballot(need);
-        // rank k of this lane's draw (k < n: base + popcount stays within the
-        // nr (NT - 1) draws of the round)
-        const int k = base + __popcll(m & lt);
-        // u < exp(delta) decided from lu = logf((float)u), |lu - log u| < 1e-5
-        // (u rounded to FP32, then logf within a few ulp of |lu| <= 34; u = 0
-        // gives -inf): outside a 3e-5 band around delta the order of lu and
-        // delta is that of u and exp(delta).  Below delta = -700 every u > 0
-        // (at least 2^-48) exceeds exp(delta); that case goes to FP64 too.
-        const double dd = delta - (double)L.sLU[k];
-        // (bitwise: no short-circuit branches)
-        bool accept = valid & (!need | (dd > 3e-5));
-        const bool band = need & ((delta < -700.0) | (fabs(dd) <= 3e-5));
-        // about 6e-5 of draws: the decision in FP64 on the draw itself (kept
-        // off the fall-through path, which every level takes)
-        if (__builtin_expect(__ballot(band) != 0, 0)) {
-            if (band) {
-                uint64_t A, C;
-                lcg48_coeffs((uint64_t)k + 1, &A, &C);
-                accept = lcg48_unit((A * X0 + C) & kMask48) < exp_band(delta);
+    const unsigned long long vmask = __ballot(valid);
+    // Level t's inputs come as two candidates chosen by the previous decision
+    // (mask accm): A if level t-1 swapped (the replica carried into t-1 goes
+    // on), R if not (slot t's original is carried).  Level 0 has one candidate.
+    int pcA = L.sP[lr];
+    double ecA = L.sE[lr];
+    int pb = L.sP[nr + lr];  // slot t+1's original, the partner at level t
+    double eb = L.sE[nr + lr];
+    double b1 = L.sB[1];
+    double dA = (L.sB[0] - b1) * (ecA - eb);
+    unsigned long long mA = __ballot(!(dA >= 0.0)) & vmask;  // lanes needing a draw
+    int pcR = pcA;
+    double ecR = ecA, dR = dA;
+    unsigned long long mR = mA;
+    unsigned long long accm = ~0ull;
+    int base = 0;  // draws taken before this level
+    // Running LDS pointers: the slot written at level t is t - 1 (level 0
+    // writes the row before slot 0); the slot read ahead is t + 2 (the last
+    // level reads past slot NT - 1: the row after it, sEr and sacc, unused).
+    int* pw = valid ? L.sP + lane - nr : L.sP + n;
+    const int sw = valid ? nr : 0;
+    const int* pp = L.sP + 2 * nr + lr;
+    const double* pe = L.sE + 2 * nr + lr;
+    const double* pbt = L.sB + 2;
+    for (int t0 = 0; t0 < NT - 1; t0 += 64) {
+        const int t1 = NT - 1 < t0 + 64 ? NT - 1 : t0 + 64;
+        int cnt = 0;  // accepted swaps of level t0 + l, in lane l
+        for (int t = t0; t < t1; t++) {
+            // slot t+2's original, for the next level's candidates (read first,
+            // so that the wait for this level's draw covers it)
+            const int pn = *pp;
+            const double en = *pe;
+            const double b2 = *pbt;
+            pp += nr;
+            pe += nr;
+            pbt++;
+            __builtin_amdgcn_sched_barrier(0);
+            const unsigned long long m = (accm & mA) | (~accm & mR);
+            // rank k of this lane's draw (k < n: base + popcount stays within
+            // the nr (NT - 1) draws of the round)
+            const int k = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)base));
+            // the previous level's slot goes out before the draw is read: LDS
+            // completes in order, so the wait for the draw covers the store and
+            // the next level starts with nothing outstanding
+            const bool acc = __builtin_amdgcn_inverse_ballot_w64(accm);
+            *pw = acc ? pcR : pcA;
+            pw += sw;
+            __builtin_amdgcn_sched_barrier(0);
+            const float lu = L.sLU[k];
+            __builtin_amdgcn_sched_barrier(0);
+            // off the chain: the previous decision's picks and both candidates
+            // of the next level
+            const int pc = acc ? pcA : pcR;
+            const double ec = acc ? ecA : ecR;
+            const double delta = acc ? dA : dR;
+            const double nA = (b1 - b2) * (ec - en);
+            const double nR = (b1 - b2) * (eb - en);
+            const unsigned long long nmA = __ballot(!(nA >= 0.0)) & vmask;
+            const unsigned long long nmR = __ballot(!(nR >= 0.0)) & vmask;
+            // u < exp(delta) decided from lu = logf((float)u), |lu - log u| <
+            // 1e-5 (u rounded to FP32, then logf within a few ulp of |lu| <= 34;
+            // u = 0 gives -inf): outside a 3e-5 band around delta the order of
+            // lu and delta is that of u and exp(delta).  Below delta = -700
+            // every u > 0 (at least 2^-48) exceeds exp(delta); that case goes to
+            // FP64 too.
+            const double dd = delta - (double)lu;
+            unsigned long long am = (~m | __ballot(dd > 3e-5)) & vmask;
+            const unsigned long long bm = m & (__ballot(delta < -700.0) | __ballot(fabs(dd) <= 3e-5));
+            // about 6e-5 of draws: the decision in FP64 on the draw itself
+            if (__builtin_expect(bm != 0, 0)) {
+                bool accept = __builtin_amdgcn_inverse_ballot_w64(am);
+                if (__builtin_amdgcn_inverse_ballot_w64(bm)) {
+                    uint64_t A, C;
+                    lcg48_coeffs((uint64_t)k + 1, &A, &C);
+                    accept = lcg48_unit((A * X0 + C) & kMask48) < exp_band(delta);
+                }
+                am = __ballot(accept) & vmask;
             }
+            base += __popcll(m);
+            cnt = lane == t - t0 ? __popcll(am) : cnt;
+            // this level's outcomes become the next level's candidates
+            pcA = pc;
+            ecA = ec;
+            pcR = pb;
+            ecR = eb;
+            dA = nA;
+            dR = nR;
+            mA = nmA;
+            mR = nmR;
+            pb = pn;
+            eb = en;
+            b1 = b2;
+            accm = am;
         }
-        base += __popcll(m);
-        // slot t keeps the carried replica unless the pair swaps (slot t is not
-        // read again: the prefetch is two slots ahead)
-        L.sP[valid ? t * nr + lane : n] = accept ? pb : pc;
-        // a swap moves the carried replica down to slot t+1; otherwise slot
-        // t+1's replica is carried on
-        pc = accept ? pc : pb;
-        ec = accept ? ec : eb;
-        bt = b1;
-        abits |= (unsigned long long)accept << (t & 63);
-        if (__builtin_expect((t & 63) == 63 || t == NT - 2, 0)) {  // uniform: every 64 levels and at the end
-            L.sAB[(t >> 6) * 64 + lane] = abits;
-            abits = 0;
-        }
+        if (t0 + lane < t1) L.sacc[t0 + lane] = cnt;
     }
-    if (valid) L.sP[(NT - 1) * nr + lane] = pc;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // the betas first: in a multi-round launch every other workgroup is
-    // waiting for its own
-    for (int i = lane; i < n; i += 64) dev_store(a.betas + L.sP[i], L.sB[i / nr]);
-    for (int i = lane; i < n; i += 64) dev_store(a.perm + i, L.sP[i]);
-    for (int t = lane; t < NT - 1; t += 64) {  // accepted swaps per level
-        const unsigned long long* w = L.sAB + (t >> 6) * 64;
-        long long c = 0;
-        for (int r = 0; r < nr; r++) c += (w[r] >> (t & 63)) & 1ull;
-        L.sacc[t] = c;
+    const bool acc = __builtin_amdgcn_inverse_ballot_w64(accm);
+    *pw = acc ? pcR : pcA;  // slot NT - 2
+    if (valid) {
+        L.sP[(NT - 1) * nr + lane] = acc ? pcA : pcR;
+        L.sEr[lane] = acc ? ecA : ecR;  // the energy that ends in slot NT - 1
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int t = lane; t < NT - 1; t += 64) {  // the counters as staged, plus this round
+    if (lane == 0) *L.sDraws = base;
+    __builtin_amdgcn_s_setprio(0);
+}
+
+// exchange_commit writes the round out with nthr threads, after a barrier
+// that follows exchange_wave: every replica's new beta first (in a
+// multi-round launch the other workgroups wait for their own), then the
+// permutation, the exchange counters, the T_min series and the 48-bit state
+// after the draws taken.  The thread that writes replica `self`'s beta also
+// puts it in *self_beta (self < 0: none).
+__device__ inline void exchange_commit(const ExArgs& a, void* lds, int tid, int nthr, int self,
+                                       double* self_beta) {
+    asm volatile("" : "+v"(tid));  // nothing thread-derived hoisted out of a caller's loop
+    const int NT = a.NT, nr = a.nrep, n = NT * nr;
+    ExLds L(lds, NT, nr);
+    for (int i = tid; i < n; i += nthr) {
+        const int p = L.sP[i];
+        const double b = L.sB[i / nr];
+        dev_store(a.betas + p, b);
+        if (p == self) *self_beta = b;
+    }
+    for (int i = tid; i < n; i += nthr) dev_store(a.perm + i, L.sP[i]);
+    for (int t = tid; t < NT - 1; t += nthr) {  // the counters as staged, plus this round
         long long* ex = a.ex;
         const long long k = L.sacc[t];
         dev_store(ex + t, L.sctr[t] + k);
@@ -240,14 +307,13 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
         dev_store(ex + 2 * (NT - 1) + t, L.sctr[2 * (NT - 1) + t] + k);
         dev_store(ex + 3 * (NT - 1) + t, L.sctr[3 * (NT - 1) + t] + nr);
     }
-    if (a.series) {
-        const long long round = *L.sser;
-        if (valid) dev_store(a.series + (size_t)(round % a.series_cap) * nr + lane, ec);
-    }
-    if (lane == 0) {
-        uint64_t A, C;  // the state after the round's base draws
-        lcg48_coeffs((uint64_t)base, &A, &C);
-        dev_store(a.rng48, (A * X0 + C) & kMask48);
+    if (a.series)
+        for (int r = tid; r < nr; r += nthr)
+            dev_store(a.series + (size_t)(*L.sser % a.series_cap) * nr + r, L.sEr[r]);
+    if (tid == 0) {
+        uint64_t A, C;  // the state after the round's draws
+        lcg48_coeffs((uint64_t)*L.sDraws, &A, &C);
+        dev_store(a.rng48, (A * *L.sX0 + C) & kMask48);
         if (a.series) dev_store(a.series_n, *L.sser + 1);
     }
 }

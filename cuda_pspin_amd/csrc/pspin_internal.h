@@ -324,13 +324,13 @@ struct ExArgs {
 };
 ExArgs exchange_args(const pspin_state* st);
 // LDS bytes of the one-wave exchange for NT x nrep slots (layout in ExLds,
-// exchange_dev.h): energies by slot and by replica, draw logs and the
-// permutation per slot (one spare entry); betas, accept counts and the four
-// exchange counters per level; the 48-bit state and the series counter; the
-// accept bits, one 64-bit word per lane and 64 levels.
+// exchange_dev.h): energies by slot and by replica, draw logs, and the
+// permutation per slot with a spare row on each side; betas, accept counts and
+// the four exchange counters per level; the 48-bit state, the series counter
+// and the number of draws taken.
 inline size_t exchange_lds_bytes(int NT, int nrep) {
-    return (size_t)NT * nrep * (2 * sizeof(double) + sizeof(float) + sizeof(int)) + sizeof(int) +
-           (size_t)NT * 6 * sizeof(double) + 2 * sizeof(double) + (size_t)(NT - 1 + 63) / 64 * 64 * sizeof(uint64_t);
+    return (size_t)(NT + 2) * nrep * sizeof(int) + (size_t)NT * nrep * (2 * sizeof(double) + sizeof(float)) +
+           (size_t)NT * 6 * sizeof(double) + 3 * sizeof(double);
 }
 // LDS bytes of the many-ladder exchange kernel (more than 64 ladders).
 inline size_t exchange_wide_lds_bytes(int NT, int nrep) {

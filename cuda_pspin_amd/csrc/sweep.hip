@@ -763,10 +763,11 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
             __syncthreads();
             if (w == 0) exchange_wave(a.ex, &view[0][0], lane);
             else if (more) draw_next();
+            __syncthreads();
+            // every thread writes the round out; this replica's new beta comes back in beta_new
+            exchange_commit(a.ex, &view[0][0], tid, bdim, rep, &beta_new);
             if (more) {
-                __syncthreads();  // wave 0's beta stores are done: read back this replica's below
-                if (tid == 0) beta_new = dev_load(a.betas + rep);
-                // the exchange staged its slots over the amplitude copies
+                __syncthreads();  // the exchange's LDS is read: the amplitude copies go back over it
                 for (int i = lane; i < ((N + 3) & ~3); i += 64)
                     s32[i] = i < N ? make_float2((float)s64[i].x, (float)s64[i].y) : make_float2(0.f, 0.f);
             }
