@@ -72,7 +72,7 @@ void free_all(pspin_state* st) {
     dfree(st->d_echunk); dfree(st->d_epart);
     if (st->graph) (void)hipGraphExecDestroy(st->graph);
     st->graph = nullptr;
-    dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48); dfree(st->d_done);
+    dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48); dfree(st->d_done); dfree(st->d_hand);
     dfree(st->d_series); dfree(st->d_series_n); dfree(st->d_log);
     dfree(st->d_ptab); dfree(st->d_pinfo);
     dfree(st->d_tasks); dfree(st->d_wave_tasks);
@@ -1016,7 +1016,7 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
                "too many replicas for the exchange kernel (NT * nrep_per_temp <= about 2100 with up to 64 "
                "ladders, 5000 with more)");
     drop_graph(st);
-    dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48); dfree(st->d_done);
+    dfree(st->d_beta_sched); dfree(st->d_perm); dfree(st->d_ex); dfree(st->d_rng48); dfree(st->d_done); dfree(st->d_hand);
     st->pt_NT = NT;
     st->pt_nrep = nrep_per_temp;
     st->pt_sweeps = 0;
@@ -1026,6 +1026,9 @@ int pspin_pt_init(pspin_state* st, int NT, int nrep_per_temp, const double* beta
     PS_TRY_HIP(dmalloc(&st->d_rng48, 1));
     PS_TRY_HIP(dmalloc(&st->d_done, 3));
     PS_TRY_HIP(fill0(st, st->d_done, 3 * sizeof(unsigned int)));
+    PS_TRY_HIP(dmalloc(&st->d_hand, (size_t)3 * NT * nrep_per_temp));
+    PS_TRY_HIP(fill0(st, st->d_hand, (size_t)3 * NT * nrep_per_temp * sizeof(unsigned long long)));
+    st->hand_tag = 0;
     std::vector<int> perm((size_t)NT * nrep_per_temp);
     std::vector<double> betas((size_t)NT * nrep_per_temp);
     for (int i = 0; i < NT * nrep_per_temp; i++) {

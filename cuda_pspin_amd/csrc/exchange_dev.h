@@ -82,6 +82,26 @@ struct ExLds {
     }
 };
 
+// Bound on the polls of a tagged handoff word (s_sleep 2 between polls, about
+// a second in all).
+constexpr unsigned kHandoffSpinLimit = 1u << 23;
+
+// Replica r's energy from its two tagged words (see exchange_stage).
+__device__ inline double tagged_energy(const unsigned long long* ew, int r, unsigned tag, unsigned* expired) {
+    unsigned long long lo = 0, hi = 0;
+    for (unsigned n = 0;; n++) {
+        lo = dev_load(ew + 2 * r);
+        hi = dev_load(ew + 2 * r + 1);
+        if ((unsigned)(lo >> 32) == tag && (unsigned)(hi >> 32) == tag) break;
+        if (n == kHandoffSpinLimit) {
+            __hip_atomic_fetch_or(expired, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+}
+
 // exchange_stage fills the slots with nthr threads.  Everything the round
 // reads from global memory is requested at once, in one round trip: the
 // permutation, every replica's energy (by replica, so no load waits for the
@@ -90,7 +110,15 @@ struct ExLds {
 // can take.  The energies are then put in slot order in LDS.  It contains a
 // workgroup barrier, so all nthr threads call it; the caller then orders LDS
 // (barrier) before exchange_wave.
-__device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int nthr) {
+//
+// With ew, the energies come as tagged words instead of from a.E: replica r
+// publishes {tag, low half} in ew[2r] and {tag, high half} in ew[2r + 1]; a
+// thread polls a replica's two words until both carry this round's tag (a
+// bounded wait: past kHandoffSpinLimit polls it sets *expired and takes what
+// it read).
+__device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int nthr,
+                                      const unsigned long long* ew = nullptr, unsigned tag = 0,
+                                      unsigned* expired = nullptr) {
     asm volatile("" : "+v"(tid));  // nothing thread-derived hoisted out of a caller's loop
     const int NT = a.NT, nr = a.nrep, n = NT * nr, nd = nr * (NT - 1), nc = 4 * (NT - 1);
     ExLds L(lds, NT, nr);
@@ -102,7 +130,7 @@ __device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int n
     for (int k = 0; k < 4; k++) {
         const int i = tid + k * nthr;
         p[k] = i < n ? dev_load(a.perm + i) : 0;
-        e[k] = i < n ? dev_load(a.E + i) : 0.0;
+        e[k] = i < n && !ew ? dev_load(a.E + i) : 0.0;
     }
     const double bs = tid < NT ? a.beta_sched[tid] : 0.0;
     long long c[2];
@@ -122,6 +150,10 @@ __device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int n
             x = (An * x + Cn) & kMask48;
         }
     }
+    if (ew)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (tid + k * nthr < n) e[k] = tagged_energy(ew, tid + k * nthr, tag, expired);
 #pragma unroll
     for (int k = 0; k < 4; k++)
         if (tid + k * nthr < n) {
@@ -139,7 +171,7 @@ __device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int n
     // what the first batches did not cover (small workgroups only)
     for (int i = tid + 4 * nthr; i < n; i += nthr) {
         L.sP[i] = dev_load(a.perm + i);
-        L.sEr[i] = dev_load(a.E + i);
+        L.sEr[i] = ew ? tagged_energy(ew, i, tag, expired) : dev_load(a.E + i);
     }
     for (int i = tid + nthr; i < NT; i += nthr) L.sB[i] = a.beta_sched[i];
     for (int i = tid + 2 * nthr; i < nc; i += nthr) L.sctr[i] = dev_load(a.ex + i);
@@ -286,15 +318,19 @@ __device__ inline void exchange_wave(const ExArgs& a, void* lds, int lane) {
 // multi-round launch the other workgroups wait for their own), then the
 // permutation, the exchange counters, the T_min series and the 48-bit state
 // after the draws taken.  The thread that writes replica `self`'s beta also
-// puts it in *self_beta (self < 0: none).
+// puts it in *self_beta (self < 0: none).  With lvw, each replica's new level
+// goes out first, as one word {tag, level} in lvw[replica].
 __device__ inline void exchange_commit(const ExArgs& a, void* lds, int tid, int nthr, int self,
-                                       double* self_beta) {
+                                       double* self_beta, unsigned long long* lvw = nullptr,
+                                       unsigned tag = 0) {
     asm volatile("" : "+v"(tid));  // nothing thread-derived hoisted out of a caller's loop
     const int NT = a.NT, nr = a.nrep, n = NT * nr;
     ExLds L(lds, NT, nr);
     for (int i = tid; i < n; i += nthr) {
         const int p = L.sP[i];
-        const double b = L.sB[i / nr];
+        const int t = i / nr;
+        if (lvw) dev_store(lvw + p, ((unsigned long long)tag << 32) | (unsigned)t);
+        const double b = L.sB[t];
         dev_store(a.betas + p, b);
         if (p == self) *self_beta = b;
     }

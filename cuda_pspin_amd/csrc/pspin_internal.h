@@ -269,6 +269,9 @@ struct pspin_state {
     int series_cap = 0;
     long long* d_series_n = nullptr;
     unsigned int* d_done = nullptr;  // [3]: workgroups arrived, unused, barrier wait expired
+    // tagged handoff words of cooperative launches: [2 nrep] energy halves, [nrep] new levels
+    unsigned long long* d_hand = nullptr;
+    unsigned hand_tag = 0;  // tag base of the next cooperative launch (grows by its rounds)
 
     // step log
     ps::StepRec* d_log = nullptr;

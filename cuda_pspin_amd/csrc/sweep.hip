@@ -55,6 +55,8 @@ constexpr unsigned kBarrierSpinLimit = 1u << 23;
 // What a workgroup leaves in its beta slot before arriving at the grid
 // barrier: a NaN bit pattern, never a beta (betas are finite and >= 0).
 constexpr unsigned long long kBetaMarker = 0x7FF8BE7A0DDC0DE5ull;
+// Temperatures a tagged launch keeps in LDS (a new level is looked up there).
+constexpr int kTagMaxNT = 64;
 
 struct SweepArgs {
     int N, nrep, h2_active, n_sweeps;
@@ -84,6 +86,12 @@ struct SweepArgs {
     int exchange;
     unsigned int* done;     // [0] workgroups arrived, [1] unused, [2] barrier wait expired
     ExArgs ex;
+    // cooperative launches with NT <= kTagMaxNT: workgroup 0 exchanges, and
+    // energies and new levels move as tagged words (tag of round rd: tag0 + rd + 1)
+    int tagged;
+    unsigned tag0;
+    unsigned long long* ew;   // [2 nrep] energy halves {tag, half}
+    unsigned long long* lvw;  // [nrep] new level {tag, level}
     const int4* tasks;        // row batches {slot | first << 1 | last << 2 | rows << 8, first pair, first row, tile offset}
     const int* wave_tasks;    // [nwaves+1] batch ranges per wave (at most 64 each)
     // GROUPS layout
@@ -513,6 +521,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
     float(*part)[16] = ss.part;
     __shared__ int last_wg;
     __shared__ double beta_new;  // the beta after an exchange, read by thread 0
+    __shared__ double sbs[kTagMaxNT];  // tagged launches: the temperatures by level
     __shared__ int goff[kMaxN + 1];  // GROUPS: chunk offsets per site
     extern __shared__ float g2s[];  // [N][N] two-body couplings when g2_lds
 
@@ -541,6 +550,8 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         for (int i = tid; i < N * N; i += bdim) g2s[i] = a.g2[i];
     if (GROUPS)
         for (int i = tid; i <= N; i += bdim) goff[i] = a.grp_off[i];
+    if (a.tagged)
+        for (int i = tid; i < a.ex.NT; i += bdim) sbs[i] = a.ex.beta_sched[i];
     // GROUPS: chunks are dealt from the last thread down, so a partial last
     // batch leaves wave 0 (which also forms the next proposal) the least work
     const int gt = bdim - 1 - tid;
@@ -727,7 +738,8 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
         __syncthreads();  // pre[] and prop[] are rewritten by the next sweep
     }
     if (a.exchange) {
-        // Last workgroup out runs the replica exchange on every energy.  Between
+        // Without tagged handoffs (below), the last workgroup out runs the
+        // replica exchange on every energy.  Between
         // rounds of a cooperative launch the others wait for their new beta
         // (a grid barrier: an arrival count, and each replica's beta slot).
         // Ordering across workgroups follows the HIP memory model at agent
@@ -750,48 +762,98 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 for (int k = lane; k < nsteps; k += 64)
                     make_step_part<CUBE>(pre[k], part, seed, gkey, sweep_n, k, N, beta, true);
         };
-        if (tid == 0) {
-            dev_store(a.E + rep, E);
-            if (more) dev_store((unsigned long long*)(a.betas + rep), kBetaMarker);
-            last_wg = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
-            if (last_wg) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-        __syncthreads();
-        if (last_wg) {
-            if (tid == 0) dev_store(a.done, 0u);
-            exchange_stage(a.ex, &view[0][0], tid, bdim);
-            __syncthreads();
-            if (w == 0) exchange_wave(a.ex, &view[0][0], lane);
-            else if (more) draw_next();
-            __syncthreads();
-            // every thread writes the round out; this replica's new beta comes back in beta_new
-            exchange_commit(a.ex, &view[0][0], tid, bdim, rep, &beta_new);
-            if (more) {
-                __syncthreads();  // the exchange's LDS is read: the amplitude copies go back over it
-                for (int i = lane; i < ((N + 3) & ~3); i += 64)
-                    s32[i] = i < N ? make_float2((float)s64[i].x, (float)s64[i].y) : make_float2(0.f, 0.f);
-            }
-        } else if (more) {
+        if (a.tagged) {
+            // Tagged handoffs (cooperative launches, every workgroup resident):
+            // each workgroup publishes its energy as two words {tag, half};
+            // workgroup 0 gathers them, runs the exchange and publishes each
+            // replica's new level as one word {tag, level}.  A word carries
+            // all that is read from it, so no ordering between words is
+            // needed: a reader that sees this round's tag has this round's
+            // value.  The exchange state (permutation, counters, LCG state,
+            // series) is written and read by workgroup 0 alone.
+            const unsigned tag = a.tag0 + (unsigned)rd + 1u;
             if (tid == 0) {
-                // Bounded wait: every workgroup is resident (cooperative launch), so
-                // the new beta comes; past the bound the kernel flags it and keeps
-                // the old beta.
-                double b = beta;
-                for (unsigned n = 0;; n++) {
-                    const unsigned long long v = dev_load((const unsigned long long*)(a.betas + rep));
-                    if (v != kBetaMarker) {
-                        b = __longlong_as_double((long long)v);
-                        break;
-                    }
-                    if (n == kBarrierSpinLimit) {
-                        __hip_atomic_fetch_or(a.done + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
+                const unsigned long long eb = (unsigned long long)__double_as_longlong(E);
+                dev_store(a.ew + 2 * rep, ((unsigned long long)tag << 32) | (eb & 0xffffffffull));
+                dev_store(a.ew + 2 * rep + 1, ((unsigned long long)tag << 32) | (eb >> 32));
+            }
+            if (rep == 0) {
+                exchange_stage(a.ex, &view[0][0], tid, bdim, a.ew, tag, a.done + 2);
+                __syncthreads();
+                if (w == 0) exchange_wave(a.ex, &view[0][0], lane);
+                else if (more) draw_next();
+                __syncthreads();
+                exchange_commit(a.ex, &view[0][0], tid, bdim, rep, &beta_new, a.lvw, tag);
+                if (more) {
+                    __syncthreads();  // the exchange's LDS is read: the amplitude copies go back over it
+                    for (int i = lane; i < ((N + 3) & ~3); i += 64)
+                        s32[i] = i < N ? make_float2((float)s64[i].x, (float)s64[i].y) : make_float2(0.f, 0.f);
                 }
-                beta_new = b;
-            } else if (w > 0) {
-                draw_next();
+            } else if (more) {
+                if (tid == 0) {
+                    // Bounded wait, as below.
+                    double b = beta;
+                    for (unsigned n = 0;; n++) {
+                        const unsigned long long v = dev_load(a.lvw + rep);
+                        if ((unsigned)(v >> 32) == tag) {
+                            b = sbs[(unsigned)v];
+                            break;
+                        }
+                        if (n == kBarrierSpinLimit) {
+                            __hip_atomic_fetch_or(a.done + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                    beta_new = b;
+                } else if (w > 0) {
+                    draw_next();
+                }
+            }
+        } else {
+            if (tid == 0) {
+                dev_store(a.E + rep, E);
+                if (more) dev_store((unsigned long long*)(a.betas + rep), kBetaMarker);
+                last_wg = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+                if (last_wg) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            __syncthreads();
+            if (last_wg) {
+                if (tid == 0) dev_store(a.done, 0u);
+                exchange_stage(a.ex, &view[0][0], tid, bdim);
+                __syncthreads();
+                if (w == 0) exchange_wave(a.ex, &view[0][0], lane);
+                else if (more) draw_next();
+                __syncthreads();
+                // every thread writes the round out; this replica's new beta comes back in beta_new
+                exchange_commit(a.ex, &view[0][0], tid, bdim, rep, &beta_new);
+                if (more) {
+                    __syncthreads();  // the exchange's LDS is read: the amplitude copies go back over it
+                    for (int i = lane; i < ((N + 3) & ~3); i += 64)
+                        s32[i] = i < N ? make_float2((float)s64[i].x, (float)s64[i].y) : make_float2(0.f, 0.f);
+                }
+            } else if (more) {
+                if (tid == 0) {
+                    // Bounded wait: every workgroup is resident (cooperative launch), so
+                    // the new beta comes; past the bound the kernel flags it and keeps
+                    // the old beta.
+                    double b = beta;
+                    for (unsigned n = 0;; n++) {
+                        const unsigned long long v = dev_load((const unsigned long long*)(a.betas + rep));
+                        if (v != kBetaMarker) {
+                            b = __longlong_as_double((long long)v);
+                            break;
+                        }
+                        if (n == kBarrierSpinLimit) {
+                            __hip_atomic_fetch_or(a.done + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                    beta_new = b;
+                } else if (w > 0) {
+                    draw_next();
+                }
             }
         }
         if (more) {
@@ -855,6 +917,10 @@ SweepArgs sweep_args(pspin_state* st, int n_sweeps, bool exchange, int rounds) {
     SweepArgs a;
     a.exchange = exchange && exchange_fusable(st);
     a.done = st->d_done;
+    a.tagged = 0;
+    a.tag0 = 0;
+    a.ew = st->d_hand;
+    a.lvw = st->d_hand ? st->d_hand + 2 * st->nrep : nullptr;
     if (a.exchange) a.ex = exchange_args(st);
     a.rounds = rounds;
     a.N = st->N;
@@ -916,6 +982,13 @@ bool pt_rounds_ok(const pspin_state* st) {
 
 hipError_t launch_pt_rounds(pspin_state* st, int pt_freq, int rounds) {
     SweepArgs a = sweep_args(st, pt_freq, true, rounds);
+    // Tags only grow from one launch to the next (never replayed from a
+    // graph), so a handoff word holds this round's tag or an older one.
+    a.tagged = a.exchange && st->pt_NT <= kTagMaxNT && st->d_hand;
+    if (a.tagged) {
+        a.tag0 = st->hand_tag;
+        st->hand_tag += (unsigned)rounds;
+    }
     void* args[] = {&a};
     return hipLaunchCooperativeKernel((const void*)sweep_kernel(st), dim3(st->nrep), dim3(sweep_block(st)), args,
                                       (unsigned)sweep_shm(a), st->stream);
