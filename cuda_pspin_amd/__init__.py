@@ -150,6 +150,7 @@ def load_library() -> C.CDLL:
         "pspin_get_rng_info": (C.c_int, [P, u64p, C.POINTER(C.c_uint64)]),
         "pspin_set_profiling": (C.c_int, [P, C.c_int]),
         "pspin_set_launch_mode": (C.c_int, [P, C.c_int]),
+        "pspin_set_wait_limit": (C.c_int, [P, C.c_uint]),
         "pspin_get_profile_launches": (C.c_int, [P, C.POINTER(C.c_longlong)]),
         "pspin_get_profile": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
@@ -382,6 +383,10 @@ class PSpin:
     def set_launch_mode(self, mode: int) -> None:
         """LAUNCH_AUTO (cooperative multi-round launches when they fit), LAUNCH_GRAPHS or LAUNCH_DIRECT."""
         _check(self._L.pspin_set_launch_mode(self._h, int(mode)))
+
+    def set_wait_limit(self, polls: int) -> None:
+        """Bound on the polls of a cross-workgroup wait in cooperative launches (0: the default)."""
+        _check(self._L.pspin_set_wait_limit(self._h, int(polls)))
 
     def set_profiling(self, on: bool) -> None:
         _check(self._L.pspin_set_profiling(self._h, int(bool(on))))

@@ -1182,6 +1182,12 @@ int pspin_set_launch_mode(pspin_state* st, int mode) {
     return PSPIN_OK;
 }
 
+int pspin_set_wait_limit(pspin_state* st, unsigned polls) {
+    PS_REQUIRE(st, PSPIN_ERR_ARG, "null argument");
+    st->wait_limit = polls;
+    return PSPIN_OK;
+}
+
 int pspin_get_profile(pspin_state* st, double* sweep_ms, long long* n_sweep, double* exch_ms, long long* n_exch) {
     PS_REQUIRE(st, PSPIN_ERR_ARG, "null argument");
     int rc = harvest_events(st);

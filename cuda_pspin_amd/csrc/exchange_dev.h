@@ -82,18 +82,16 @@ struct ExLds {
     }
 };
 
-// Bound on the polls of a tagged handoff word (s_sleep 2 between polls, about
-// a second in all).
-constexpr unsigned kHandoffSpinLimit = 1u << 23;
-
-// Replica r's energy from its two tagged words (see exchange_stage).
-__device__ inline double tagged_energy(const unsigned long long* ew, int r, unsigned tag, unsigned* expired) {
+// Replica r's energy from its two tagged words (see exchange_stage); at most
+// `limit` polls.
+__device__ inline double tagged_energy(const unsigned long long* ew, int r, unsigned tag, unsigned* expired,
+                                       unsigned limit) {
     unsigned long long lo = 0, hi = 0;
     for (unsigned n = 0;; n++) {
         lo = dev_load(ew + 2 * r);
         hi = dev_load(ew + 2 * r + 1);
         if ((unsigned)(lo >> 32) == tag && (unsigned)(hi >> 32) == tag) break;
-        if (n == kHandoffSpinLimit) {
+        if (n == limit) {
             __hip_atomic_fetch_or(expired, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
         }
@@ -114,11 +112,10 @@ __device__ inline double tagged_energy(const unsigned long long* ew, int r, unsi
 // With ew, the energies come as tagged words instead of from a.E: replica r
 // publishes {tag, low half} in ew[2r] and {tag, high half} in ew[2r + 1]; a
 // thread polls a replica's two words until both carry this round's tag (a
-// bounded wait: past kHandoffSpinLimit polls it sets *expired and takes what
-// it read).
+// bounded wait: past `limit` polls it sets *expired and takes what it read).
 __device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int nthr,
                                       const unsigned long long* ew = nullptr, unsigned tag = 0,
-                                      unsigned* expired = nullptr) {
+                                      unsigned* expired = nullptr, unsigned limit = 0) {
     asm volatile("" : "+v"(tid));  // nothing thread-derived hoisted out of a caller's loop
     const int NT = a.NT, nr = a.nrep, n = NT * nr, nd = nr * (NT - 1), nc = 4 * (NT - 1);
     ExLds L(lds, NT, nr);
@@ -153,7 +150,7 @@ __device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int n
     if (ew)
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (tid + k * nthr < n) e[k] = tagged_energy(ew, tid + k * nthr, tag, expired);
+            if (tid + k * nthr < n) e[k] = tagged_energy(ew, tid + k * nthr, tag, expired, limit);
 #pragma unroll
     for (int k = 0; k < 4; k++)
         if (tid + k * nthr < n) {
@@ -171,7 +168,7 @@ __device__ inline void exchange_stage(const ExArgs& a, void* lds, int tid, int n
     // what the first batches did not cover (small workgroups only)
     for (int i = tid + 4 * nthr; i < n; i += nthr) {
         L.sP[i] = dev_load(a.perm + i);
-        L.sEr[i] = ew ? tagged_energy(ew, i, tag, expired) : dev_load(a.E + i);
+        L.sEr[i] = ew ? tagged_energy(ew, i, tag, expired, limit) : dev_load(a.E + i);
     }
     for (int i = tid + nthr; i < NT; i += nthr) L.sB[i] = a.beta_sched[i];
     for (int i = tid + 2 * nthr; i < nc; i += nthr) L.sctr[i] = dev_load(a.ex + i);

@@ -272,6 +272,7 @@ struct pspin_state {
     // tagged handoff words of cooperative launches: [2 nrep] energy halves, [nrep] new levels
     unsigned long long* d_hand = nullptr;
     unsigned hand_tag = 0;  // tag base of the next cooperative launch (grows by its rounds)
+    unsigned wait_limit = 0;  // polls of a cross-workgroup wait before it is flagged (0: default)
 
     // step log
     ps::StepRec* d_log = nullptr;

@@ -50,7 +50,7 @@
 
 namespace ps {
 
-// Grid-barrier wait bound (s_sleep 2 iterations, about a second).
+// Default bound on a cross-workgroup wait (s_sleep 2 polls, about a second).
 constexpr unsigned kBarrierSpinLimit = 1u << 23;
 // What a workgroup leaves in its beta slot before arriving at the grid
 // barrier: a NaN bit pattern, never a beta (betas are finite and >= 0).
@@ -90,6 +90,7 @@ struct SweepArgs {
     // energies and new levels move as tagged words (tag of round rd: tag0 + rd + 1)
     int tagged;
     unsigned tag0;
+    unsigned spin_limit;      // polls of a cross-workgroup wait before it is flagged
     unsigned long long* ew;   // [2 nrep] energy halves {tag, half}
     unsigned long long* lvw;  // [nrep] new level {tag, level}
     const int4* tasks;        // row batches {slot | first << 1 | last << 2 | rows << 8, first pair, first row, tile offset}
@@ -778,7 +779,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                 dev_store(a.ew + 2 * rep + 1, ((unsigned long long)tag << 32) | (eb >> 32));
             }
             if (rep == 0) {
-                exchange_stage(a.ex, &view[0][0], tid, bdim, a.ew, tag, a.done + 2);
+                exchange_stage(a.ex, &view[0][0], tid, bdim, a.ew, tag, a.done + 2, a.spin_limit);
                 __syncthreads();
                 if (w == 0) exchange_wave(a.ex, &view[0][0], lane);
                 else if (more) draw_next();
@@ -799,7 +800,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                             b = sbs[(unsigned)v];
                             break;
                         }
-                        if (n == kBarrierSpinLimit) {
+                        if (n == a.spin_limit) {
                             __hip_atomic_fetch_or(a.done + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
                         }
@@ -844,7 +845,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a) {
                             b = __longlong_as_double((long long)v);
                             break;
                         }
-                        if (n == kBarrierSpinLimit) {
+                        if (n == a.spin_limit) {
                             __hip_atomic_fetch_or(a.done + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
                         }
@@ -919,6 +920,7 @@ SweepArgs sweep_args(pspin_state* st, int n_sweeps, bool exchange, int rounds) {
     a.done = st->d_done;
     a.tagged = 0;
     a.tag0 = 0;
+    a.spin_limit = st->wait_limit ? st->wait_limit : kBarrierSpinLimit;
     a.ew = st->d_hand;
     a.lvw = st->d_hand ? st->d_hand + 2 * st->nrep : nullptr;
     if (a.exchange) a.ex = exchange_args(st);

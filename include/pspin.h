@@ -297,10 +297,12 @@ int pspin_get_profile_launches(pspin_state* st, long long* n_sweep_launches);
 
 /* How pspin_pt_run launches whole PT rounds (pt_freq sweeps, one exchange).
  * No counterpart in the reference (its host loop launches each sweep):
- *   PSPIN_LAUNCH_AUTO    one cooperative launch of many rounds that meet at a
- *                        grid barrier, the exchange run by the last workgroup
- *                        to arrive, when every workgroup fits on the device at
- *                        once; otherwise as PSPIN_LAUNCH_GRAPHS;
+ *   PSPIN_LAUNCH_AUTO    one cooperative launch of many rounds, when every
+ *                        workgroup fits on the device at once: workgroup 0
+ *                        exchanges on energies published as tagged words and
+ *                        publishes each replica's new level the same way (up
+ *                        to 64 temperatures; with more, the last workgroup to
+ *                        arrive exchanges); otherwise as PSPIN_LAUNCH_GRAPHS;
  *   PSPIN_LAUNCH_GRAPHS  captured graphs of one sweep launch per round;
  *   PSPIN_LAUNCH_DIRECT  one sweep launch per round from the host.
  * The exchange runs in the last workgroup of the round's sweep launch when up
@@ -308,6 +310,13 @@ int pspin_get_profile_launches(pspin_state* st, long long* n_sweep_launches);
  * the same chains bit for bit. */
 enum { PSPIN_LAUNCH_AUTO = 0, PSPIN_LAUNCH_GRAPHS = 1, PSPIN_LAUNCH_DIRECT = 2 };
 int pspin_set_launch_mode(pspin_state* st, int mode);
+
+/* Bound on the polls of every cross-workgroup wait in a cooperative launch
+ * (0: the default, about a second of polling).  A wait that reaches it is
+ * flagged, the launch goes on with what it has, and pspin_pt_run returns
+ * PSPIN_ERR_HIP.  For tests of that error path; no counterpart in the
+ * reference, whose rounds meet at kernel boundaries. */
+int pspin_set_wait_limit(pspin_state* st, unsigned polls);
 
 /* cudaMemGetInfo / cudaGetDeviceProperties counterpart for the drivers'
  * memory pre-check and banner (src/parallel_tempering.cu:178-202).  dev < 0

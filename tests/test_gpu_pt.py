@@ -170,6 +170,29 @@ def test_cooperative_rounds_match_kernel_boundary_exchange(pspin, oracle):
         st.close()
 
 
+@pytest.mark.parametrize("NT,nr", [(16, 4), (80, 1)])
+def test_cross_workgroup_wait_bound_is_reported(pspin, oracle, NT, nr):
+    # Cooperative launches whose cross-workgroup waits may poll only twice:
+    # the rounds cannot meet in time, every wait ends at its bound, and
+    # pt_run reports the expired wait instead of returning normally.  The
+    # default bound gives a normal run.  16 temperatures use the tagged
+    # handoffs (workgroup 0 exchanges), 80 the arrival count (the last
+    # workgroup to arrive exchanges).
+    T, beta = oracle.temperatures(NT, 0.5, 1.5, 1)
+    for limit in (1, 0):
+        st = pspin.PSpin(pspin.SimConfig.default(N=16, nrep=NT * nr, seed=5), 0)
+        st.pt_init(NT, nr, beta, 9)
+        st.set_wait_limit(limit)
+        if limit:
+            with pytest.raises(pspin.PSpinError, match="expired"):
+                st.pt_run(64, 1)
+        else:
+            st.pt_run(64, 1)
+            E, acc, prop = st.results()
+            assert np.all(np.isfinite(E)) and prop.sum() > 0
+        st.close()
+
+
 def test_periodic_resync(pspin, oracle):
     cfg = pspin.SimConfig.default(N=14, nrep=4, seed=77)
     st = pspin.PSpin(cfg, 0)
