@@ -90,7 +90,11 @@ def test_pt_run_counts_and_betas(pspin, oracle):
     st.close()
 
 
-@pytest.mark.parametrize("N,NT,nr,kernel", [(16, 4, 2, 0), (64, 32, 8, 2), (32, 8, 4, 3)])
+# (the 64- and 65-temperature cases sit on either side of the largest ladder
+# that cooperative launches hand over by tagged words; 2 temperatures is the
+# shortest ladder)
+@pytest.mark.parametrize("N,NT,nr,kernel", [(16, 4, 2, 0), (64, 32, 8, 2), (32, 8, 4, 3), (16, 64, 2, 0),
+                                            (16, 65, 1, 0), (16, 2, 16, 0)])
 def test_launch_modes_give_identical_chains(pspin, oracle, N, NT, nr, kernel):
     # Whole PT rounds run as one cooperative launch with a grid barrier (auto),
     # as captured graphs, or as direct launches (with the exchange in the
