@@ -47,6 +47,80 @@ def test_exchange_bit_exact(pspin, oracle, NT, nrep):
     st.close()
 
 
+def _band_energies(NT, nrep, beta, perm, x, rng):
+    """Energies by replica for which every exchange pair of the next round
+    takes a draw and its delta lies within 2e-5 of log u of the very draw it
+    takes (or below -700): every decision of the round goes to the FP64 test
+    on the draw itself.  x is the 48-bit drand48 state before the round; the
+    construction follows the reference's sequential rule (levels in order,
+    ladders in order within a level, a draw only for delta < 0)."""
+    mask = (1 << 48) - 1
+    E = np.zeros(NT * nrep)
+    ec = rng.normal(size=nrep)
+    E[perm[:nrep]] = ec
+    band = 0
+    for t in range(NT - 1):
+        db = beta[t] - beta[t + 1]
+        for r in range(nrep):
+            x = (0x5DEECE66D * x + 0xB) & mask
+            u = x / 2.0 ** 48
+            if rng.random() < 0.05:
+                d = -800.0
+            else:
+                # (not exactly log u: at an exact tie the FP64 exponential and
+                # glibc's may round a last bit apart, which decides the tie)
+                d = (np.log(u) if u > 0 else -1.0) + rng.choice([2e-5, -2e-5, 1e-6, -1e-6, 1e-9, -1e-9])
+                if d >= 0:
+                    d = -1.0
+            eb = ec[r] - d / db
+            E[perm[(t + 1) * nrep + r]] = eb
+            delta = db * (ec[r] - eb)
+            band += (delta < -700.0) or abs(delta - np.log(u)) <= 3e-5
+            if not (delta >= 0 or u < np.exp(delta)):
+                ec[r] = eb
+    return E, band
+
+
+@pytest.mark.parametrize("NT,nrep", [(8, 4), (32, 8), (5, 100)])
+def test_exchange_band_decisions_bit_exact(pspin, oracle, NT, nrep):
+    # The exchange decides u < exp(delta) from logf of the draw outside a
+    # 3e-5 band and by the FP64 test on the draw itself inside it (and below
+    # delta = -700); random energies reach the band for about 6e-5 of the
+    # draws.  Here the energies are built so that (nearly) every pair of
+    # every round is a band case, and the GPU must still match the oracle
+    # bit for bit (permutation, counters, drand48 state).  100 ladders run
+    # the many-ladder kernel.
+    print(f"[exchange band] NT={NT} nrep={nrep}", flush=True)
+    st = pspin.PSpin(pspin.SimConfig.default(N=8, nrep=NT * nrep, seed=5), 0)
+    T, beta = oracle.temperatures(NT, 0.2, 1.6, 1)
+    seed48 = 1234 + NT
+    st.pt_init(NT, nrep, beta, seed48)
+    oracle.srand48(seed48)
+    perm = np.arange(NT * nrep, dtype=np.int32)
+    acc = np.zeros(NT - 1, dtype=np.int64)
+    prop = np.zeros(NT - 1, dtype=np.int64)
+    rng = np.random.default_rng(NT * 7 + nrep)
+    x = st.pt_rng48()
+    band = pairs = 0
+    for rnd in range(10):
+        E, b = _band_energies(NT, nrep, beta, perm, x, rng)
+        band += b
+        pairs += (NT - 1) * nrep
+        st.set_energies(E)
+        st.pt_exchange()
+        oracle.pt_exchange(NT, nrep, beta, E, perm, acc, prop)
+        np.testing.assert_array_equal(st.pt_perm(), perm, err_msg=f"round {rnd}")
+        x = st.pt_rng48()
+    acc_w, prop_w, acc_t, prop_t = st.pt_exchange_stats()
+    np.testing.assert_array_equal(acc_t, acc)
+    np.testing.assert_array_equal(prop_t, prop)
+    nxt = (0x5DEECE66D * x + 0xB) & ((1 << 48) - 1)
+    assert oracle.drand48() == nxt / 2.0 ** 48
+    print(f"band cases {band} of {pairs} pairs", flush=True)
+    assert band >= pairs // 2
+    st.close()
+
+
 def test_reference_replica_exchange(pspin, oracle):
     # src/test_replica_exchange.cu with run_tests.sh "8 6 65536 5" (seed 65432,
     # linear T in [0.5, 2], srand48(seed+7777)); 16380 sweeps here.
